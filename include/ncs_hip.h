@@ -1,0 +1,164 @@
+/*
+ * ncs_hip.h -- C ABI of libncs_hip.so, the MI355X (gfx950) NCS solver.
+ *
+ * Plain pointers and sizes only.  Every entry point returns 0 on success and
+ * a negative NCS_E* code on error (the message is kept for
+ * ncs_hip_last_error()); none of them prints.  Per-tile solver status uses the
+ * codes of the reference GPU kernel (opencl/ncs.cl:66-71).
+ *
+ * Reference interfaces replaced (file:line in HuanglabPurdue/NCS):
+ *   ncsReduceNoise (C ABI)          clib/pyCNCS/ncs.h:42, ncs.c:41-147
+ *   ncsReduceNoise (GPU kernel ABI) opencl/ncs.cl:820-826, cuda/ncs.cu:873-879
+ *   reduceNoise (Python host)       opencl/pyOpenCLNCS/ncs.py:47-198, 236-251
+ *   setOTFMask                      opencl/pyOpenCLNCS/ncs.py:208-222,
+ *                                   clib/pyCNCS/ncs.c:502-510
+ */
+#ifndef NCS_HIP_H
+#define NCS_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* error codes */
+#define NCS_OK 0
+#define NCS_E_ARG (-1)      /* invalid argument */
+#define NCS_E_HIP (-2)      /* HIP runtime error */
+#define NCS_E_NOMASK (-3)   /* no OTF mask set on the context */
+#define NCS_E_MASK (-4)     /* mask not point-symmetric */
+#define NCS_E_NOWEIGHTS (-5) /* learned start requested without weights */
+
+/* per-tile status (opencl/ncs.cl:66-71) */
+#define NCS_SUCCESS 0
+#define NCS_UNSTARTED (-1)
+#define NCS_REACHED_MAXITERS (-2)
+#define NCS_INCREASING_GRADIENT (-3)
+#define NCS_MINIMUM_STEP (-4)
+#define NCS_REACHED_MAXPRECISION (-5)
+
+/* tiling of a frame */
+#define NCS_GEOM_REFERENCE 0 /* 16x16 windows, stride 14, 1-px replicated halo,
+                                last window clamped, interiors stitched with the
+                                later window owning overlaps (ncs.c:71-143) */
+#define NCS_GEOM_BLOCKS 1    /* non-overlapping 16x16 blocks; pixels outside the
+                                frame are zero-padded and excluded from the
+                                likelihood term */
+
+/* line searches */
+#define NCS_LS_MORE_THUENTE 0 /* liblbfgs default (lbfgs.h:704-719) */
+#define NCS_LS_ARMIJO 1       /* backtracking of opencl/ncs.cl:945-1035 */
+
+/* starting point */
+#define NCS_START_PLAIN 0   /* u0 = data (ncs.c:549-551, ncs.cl:906) */
+#define NCS_START_TILES 1   /* u0 from a caller tile buffer [tiles][256] */
+#define NCS_START_LEARNED 2 /* u0 from the context's conv net */
+
+typedef struct {
+  int max_iterations;  /* L-BFGS iteration cap (ncs.cl:61 MAXITERS = 200) */
+  int history;         /* correction pairs m, 1..8 (lbfgs.h default 6, ncs.cl 8) */
+  float epsilon;       /* stop when |g| <= epsilon * max(1, |u|); 0 disables */
+  int linesearch;      /* NCS_LS_* */
+  int max_linesearch;  /* trials per iteration */
+  float ftol;          /* sufficient decrease (Armijo c1 = ftol) */
+  float gtol;          /* curvature (More-Thuente) */
+  float xtol;          /* relative interval width (More-Thuente) */
+  float min_step;      /* smallest step */
+  float max_step;      /* largest step */
+  float stepm;         /* Armijo backtracking factor */
+  int start;           /* NCS_START_* */
+  int fallback;        /* warm starts: keep u0 = data where f(data) <= f(u0) */
+  int waves_per_cu;    /* persistent grid size; 0 = autotuned */
+} ncs_params;
+
+typedef struct {
+  const float *data;  /* photo-electrons, row-major [height][stride] */
+  const float *gamma; /* read-noise variance / gain^2, same shape */
+  float *out;         /* corrected image, same shape (may be NULL) */
+  int height, width;
+  int data_stride, gamma_stride, out_stride; /* elements; 0 = width */
+  float alpha;        /* NCS alpha of this frame */
+  int geometry;       /* NCS_GEOM_* */
+} ncs_frame;
+
+/* Context: device, stream, OTF-mask tables, work buffers, captured graphs,
+ * autotune cache and (optionally) conv-net weights. */
+typedef struct ncs_ctx ncs_ctx;
+
+/* Fill defaults: max_iterations 200, history 5, epsilon 1e-5, More-Thuente
+ * (ftol 1e-4, gtol 0.9, xtol 1e-7, max_linesearch 20, steps 1e-20..1e20),
+ * Armijo stepm 0.5, plain start, fallback on, autotuned grid. */
+void ncs_hip_default_params(ncs_params *p);
+
+int ncs_hip_create(int device, ncs_ctx **out);
+void ncs_hip_destroy(ncs_ctx *ctx);
+/* Message of the last error on this thread. */
+const char *ncs_hip_last_error(void);
+/* Version string of the library and the GPU it runs on. */
+int ncs_hip_device_info(ncs_ctx *ctx, char *buf, int buflen);
+
+/* Set the 16x16 OTF mask, FFT order, unsquared (replaces ncsSRSetOTFMask,
+ * ncs.c:502-510, and the otf_mask argument of ncs.cl:822).  The solver uses
+ * the point-symmetric part of q = mask^2, which equals q for every mask that
+ * passes the reference's checkOTFMask. */
+int ncs_hip_set_otf_mask(ncs_ctx *ctx, const float *mask_fft_order);
+
+/* Number of tiles a frame produces with the given geometry. */
+long long ncs_hip_tile_count(int height, int width, int geometry);
+
+/* Whole frames, host memory in and out; synchronous.  Replaces the batched
+ * host reduceNoise (pyOpenCLNCS/ncs.py:47-198): every frame may have its own
+ * size, gamma, alpha and geometry and all tiles of all frames go through one
+ * persistent launch.  iterations/status/cost (host, may be NULL) receive one
+ * entry per tile, frames in order, tiles in raster order. */
+int ncs_hip_reduce_frames(ncs_ctx *ctx, const ncs_frame *frames, int n_frames,
+                          const ncs_params *params, int *iterations,
+                          int *status, float *cost);
+
+/* Same with device pointers (frames[] itself in host memory), enqueued on
+ * `stream` (a hipStream_t; NULL = the context's stream) without
+ * synchronisation.  u0_tiles: device [tiles][256] for NCS_START_TILES, else
+ * NULL.  iterations/status/cost: device arrays or NULL. */
+int ncs_hip_reduce_frames_device(ncs_ctx *ctx, const ncs_frame *frames,
+                                 int n_frames, const ncs_params *params,
+                                 const float *u0_tiles, int *iterations,
+                                 int *status, float *cost, void *stream);
+
+/* Independent 16x16 tiles [num_tiles][16][16], host memory; synchronous.
+ * Replaces the kernel ABI ncsReduceNoise(data_in, g_gamma, otf_mask,
+ * data_out, g_iterations, g_status, alpha) of opencl/ncs.cl:820-826 (the
+ * mask comes from ncs_hip_set_otf_mask).  cost may be NULL. */
+int ncs_hip_solve_tiles(ncs_ctx *ctx, const float *data_in,
+                        const float *gamma, float *data_out, int *iterations,
+                        int *status, float *cost, int num_tiles, float alpha,
+                        const ncs_params *params);
+
+/* Device-pointer variant of ncs_hip_solve_tiles, enqueued on `stream`.
+ * u0 (device [num_tiles][256]) or NULL for the plain start. */
+int ncs_hip_solve_tiles_device(ncs_ctx *ctx, const float *data_in,
+                               const float *gamma, const float *u0,
+                               float *data_out, int *iterations, int *status,
+                               float *cost, int num_tiles, float alpha,
+                               const ncs_params *params, void *stream);
+
+/* FP32 cost f(u) and gradient of independent tiles [num_tiles][16][16]
+ * (host memory; grad or cost may be NULL).  The evaluation the solver uses,
+ * exposed for parity with ncsSREvaluate (clib/pyCNCS/ncs.c:335-364) and the
+ * ncsSRCalc* functions it calls (ncs.c:158-299). */
+int ncs_hip_eval_tiles(ncs_ctx *ctx, const float *data, const float *gamma,
+                       const float *u, float *cost, float *grad, int num_tiles,
+                       float alpha);
+
+/* Drop-in for the C library's whole-image entry (clib/pyCNCS/ncs.h:42):
+ * same signature and semantics (reference tiling, FFT-order mask), solved on
+ * GPU 0 in FP32 with the default parameters.  r_size must be 16; on error
+ * ncs_image is left unchanged (the C prints and returns, ncs.c:55-58). */
+void ncsReduceNoise(double *ncs_image, double *image, double *gamma,
+                    double *otf_mask, double alpha, int im_x, int im_y,
+                    int r_size);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

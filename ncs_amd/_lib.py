@@ -1,0 +1,294 @@
+"""ctypes binding of libncs_hip.so (include/ncs_hip.h).
+
+Loading fails loudly when the library has not been built: there is no CPU
+fallback anywhere in the product path.
+"""
+import ctypes
+import os
+import threading
+
+import numpy
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libncs_hip.so")
+
+NCS_OK = 0
+# per-tile status codes (opencl/ncs.cl:66-71)
+SUCCESS = 0
+UNSTARTED = -1
+REACHED_MAXITERS = -2
+INCREASING_GRADIENT = -3
+MINIMUM_STEP = -4
+REACHED_MAXPRECISION = -5
+
+GEOM_REFERENCE = 0
+GEOM_BLOCKS = 1
+LS_MORE_THUENTE = 0
+LS_ARMIJO = 1
+START_PLAIN = 0
+START_TILES = 1
+START_LEARNED = 2
+NET_PARAMS = 19393
+
+
+class NCSHipError(RuntimeError):
+    pass
+
+
+class Params(ctypes.Structure):
+    """ncs_params (include/ncs_hip.h)."""
+    _fields_ = [("max_iterations", ctypes.c_int),
+                ("history", ctypes.c_int),
+                ("epsilon", ctypes.c_float),
+                ("linesearch", ctypes.c_int),
+                ("max_linesearch", ctypes.c_int),
+                ("ftol", ctypes.c_float),
+                ("gtol", ctypes.c_float),
+                ("xtol", ctypes.c_float),
+                ("min_step", ctypes.c_float),
+                ("max_step", ctypes.c_float),
+                ("stepm", ctypes.c_float),
+                ("start", ctypes.c_int),
+                ("fallback", ctypes.c_int),
+                ("waves_per_cu", ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Frame(ctypes.Structure):
+    """ncs_frame (include/ncs_hip.h)."""
+    _fields_ = [("data", ctypes.c_void_p),
+                ("gamma", ctypes.c_void_p),
+                ("out", ctypes.c_void_p),
+                ("height", ctypes.c_int),
+                ("width", ctypes.c_int),
+                ("data_stride", ctypes.c_int),
+                ("gamma_stride", ctypes.c_int),
+                ("out_stride", ctypes.c_int),
+                ("alpha", ctypes.c_float),
+                ("geometry", ctypes.c_int)]
+
+
+# Every symbol the header declares, with its ctypes signature.
+_vp, _i, _f, _ll = ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_longlong
+_ip, _fp = ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_float)
+_P = ctypes.POINTER(Params)
+_F = ctypes.POINTER(Frame)
+SIGNATURES = {
+    "ncs_hip_default_params": (None, [_P]),
+    "ncs_hip_create": (_i, [_i, ctypes.POINTER(_vp)]),
+    "ncs_hip_destroy": (None, [_vp]),
+    "ncs_hip_last_error": (ctypes.c_char_p, []),
+    "ncs_hip_device_info": (_i, [_vp, ctypes.c_char_p, _i]),
+    "ncs_hip_set_otf_mask": (_i, [_vp, _vp]),
+    "ncs_hip_tile_count": (_ll, [_i, _i, _i]),
+    "ncs_hip_reduce_frames": (_i, [_vp, _F, _i, _P, _vp, _vp, _vp]),
+    "ncs_hip_reduce_frames_device": (_i, [_vp, _F, _i, _P, _vp, _vp, _vp, _vp, _vp]),
+    "ncs_hip_solve_tiles": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _f, _P]),
+    "ncs_hip_solve_tiles_device": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _f, _P,
+                                        _vp]),
+    "ncs_hip_eval_tiles": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _i, _f]),
+    "ncsReduceNoise": (None, [_vp, _vp, _vp, _vp, ctypes.c_double, _i, _i, _i]),
+}
+
+_lib = None
+_lock = threading.Lock()
+
+
+def load():
+    """The loaded library (raises NCSHipError if it is not built)."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise NCSHipError(
+                "libncs_hip.so is not built (expected at %s); run "
+                "`python -m ncs_amd.build` or __graft_entry__.build()" % LIB_PATH)
+        lib = ctypes.cdll.LoadLibrary(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+        return lib
+
+
+def last_error():
+    return load().ncs_hip_last_error().decode(errors="replace")
+
+
+def check(rc, what):
+    if rc != NCS_OK:
+        raise NCSHipError("%s failed (%d): %s" % (what, rc, last_error()))
+
+
+def default_params(**overrides):
+    p = Params()
+    load().ncs_hip_default_params(ctypes.byref(p))
+    for k, v in overrides.items():
+        if not hasattr(p, k):
+            raise TypeError("unknown solver parameter " + repr(k))
+        setattr(p, k, v)
+    return p
+
+
+def kernel_semantics_params(**overrides):
+    """Parameters of the reference GPU kernel (opencl/ncs.cl:58-63): Armijo
+    backtracking with c1 = 1e-4, step x0.5 down to 1e-6, history 8,
+    epsilon 1e-4, at most 200 iterations."""
+    p = default_params(linesearch=LS_ARMIJO, ftol=1e-4, stepm=0.5, min_step=1e-6,
+                       history=8, epsilon=1e-4, max_iterations=200, max_linesearch=64)
+    for k, v in overrides.items():
+        setattr(p, k, v)
+    return p
+
+
+def tile_count(height, width, geometry=GEOM_REFERENCE):
+    return int(load().ncs_hip_tile_count(int(height), int(width), int(geometry)))
+
+
+def _ptr(a):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+class Context(object):
+    """One libncs_hip context (device, stream, OTF tables, buffers)."""
+
+    def __init__(self, device=0):
+        self.lib = load()
+        h = ctypes.c_void_p()
+        check(self.lib.ncs_hip_create(int(device), ctypes.byref(h)), "ncs_hip_create")
+        self.handle = h
+        self.device = device
+        self.mask_fft = None
+
+    def close(self):
+        if self.handle is not None:
+            self.lib.ncs_hip_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        buf = ctypes.create_string_buffer(256)
+        check(self.lib.ncs_hip_device_info(self.handle, buf, 256), "ncs_hip_device_info")
+        return buf.value.decode()
+
+    def set_otf_mask(self, mask_fft_order):
+        m = numpy.ascontiguousarray(mask_fft_order, dtype=numpy.float32)
+        if m.shape != (16, 16):
+            raise ValueError("the OTF mask must be 16 x 16")
+        check(self.lib.ncs_hip_set_otf_mask(self.handle, _ptr(m)), "ncs_hip_set_otf_mask")
+        self.mask_fft = m.copy()
+
+    def reduce_frames(self, datas, gammas, outs, alphas, geometry, params):
+        """Host-memory batch; returns per-tile (iterations, status, cost)."""
+        n = len(datas)
+        frames = (Frame * n)()
+        total = 0
+        keep = []
+        for i in range(n):
+            d = numpy.ascontiguousarray(datas[i], dtype=numpy.float32)
+            g = numpy.ascontiguousarray(gammas[i], dtype=numpy.float32)
+            o = outs[i]
+            if d.ndim != 2 or g.shape != d.shape or o.shape != d.shape:
+                raise ValueError("frame %d: data, gamma and out must be 2-D of one shape" % i)
+            if o.dtype != numpy.float32 or not o.flags.c_contiguous:
+                raise ValueError("out arrays must be C-contiguous float32")
+            keep += [d, g]
+            fr = frames[i]
+            fr.data, fr.gamma, fr.out = d.ctypes.data, g.ctypes.data, o.ctypes.data
+            fr.height, fr.width = d.shape
+            fr.data_stride = fr.gamma_stride = fr.out_stride = 0
+            fr.alpha = float(alphas[i])
+            fr.geometry = int(geometry[i] if hasattr(geometry, "__len__") else geometry)
+            nt = tile_count(fr.height, fr.width, fr.geometry)
+            if nt < 0:
+                raise ValueError("frame %d of shape %s is too small for this tiling"
+                                 % (i, d.shape))
+            total += nt
+        iters = numpy.zeros(total, numpy.int32)
+        status = numpy.zeros(total, numpy.int32)
+        cost = numpy.zeros(total, numpy.float32)
+        check(self.lib.ncs_hip_reduce_frames(self.handle, frames, n, ctypes.byref(params),
+                                             _ptr(iters), _ptr(status), _ptr(cost)),
+              "ncs_hip_reduce_frames")
+        return iters, status, cost
+
+    def solve_tiles(self, data, gamma, alpha, params):
+        """Independent tiles [n, 16, 16] (host); returns (u, iters, status, cost)."""
+        d = numpy.ascontiguousarray(data, dtype=numpy.float32)
+        g = numpy.ascontiguousarray(gamma, dtype=numpy.float32)
+        if d.ndim != 3 or d.shape[1:] != (16, 16) or g.shape != d.shape:
+            raise ValueError("tiles must be [n, 16, 16] with matching gamma")
+        n = d.shape[0]
+        u = numpy.zeros_like(d)
+        iters = numpy.zeros(n, numpy.int32)
+        status = numpy.zeros(n, numpy.int32)
+        cost = numpy.zeros(n, numpy.float32)
+        check(self.lib.ncs_hip_solve_tiles(self.handle, _ptr(d), _ptr(g), _ptr(u), _ptr(iters),
+                                           _ptr(status), _ptr(cost), n, float(alpha),
+                                           ctypes.byref(params)),
+              "ncs_hip_solve_tiles")
+        return u, iters, status, cost
+
+    def eval_tiles(self, data, gamma, u, alpha):
+        """FP32 cost and gradient of tiles [n, 16, 16] (host)."""
+        d = numpy.ascontiguousarray(data, dtype=numpy.float32)
+        g = numpy.ascontiguousarray(gamma, dtype=numpy.float32)
+        uu = numpy.ascontiguousarray(u, dtype=numpy.float32)
+        n = d.shape[0]
+        cost = numpy.zeros(n, numpy.float32)
+        grad = numpy.zeros_like(uu)
+        check(self.lib.ncs_hip_eval_tiles(self.handle, _ptr(d), _ptr(g), _ptr(uu), _ptr(cost),
+                                          _ptr(grad), n, float(alpha)),
+              "ncs_hip_eval_tiles")
+        return cost, grad
+
+    def solve_tiles_device(self, data_ptr, gamma_ptr, u0_ptr, out_ptr, iters_ptr,
+                           status_ptr, cost_ptr, n, alpha, params, stream_ptr=None):
+        """Device-pointer tiles (integers from torch's data_ptr()); async."""
+        check(self.lib.ncs_hip_solve_tiles_device(
+            self.handle, data_ptr, gamma_ptr, u0_ptr, out_ptr, iters_ptr, status_ptr,
+            cost_ptr, int(n), float(alpha), ctypes.byref(params), stream_ptr),
+            "ncs_hip_solve_tiles_device")
+
+    def reduce_frames_device(self, frames, params, u0_ptr=None, iters_ptr=None,
+                             status_ptr=None, cost_ptr=None, stream_ptr=None):
+        """frames: list of dicts(data, gamma, out, height, width, alpha, geometry,
+        [data_stride, gamma_stride, out_stride]) with device pointers; async."""
+        n = len(frames)
+        arr = (Frame * n)()
+        for i, f in enumerate(frames):
+            a = arr[i]
+            a.data, a.gamma, a.out = f["data"], f["gamma"], f.get("out")
+            a.height, a.width = f["height"], f["width"]
+            a.data_stride = f.get("data_stride", 0)
+            a.gamma_stride = f.get("gamma_stride", 0)
+            a.out_stride = f.get("out_stride", 0)
+            a.alpha = float(f["alpha"])
+            a.geometry = int(f.get("geometry", GEOM_REFERENCE))
+        check(self.lib.ncs_hip_reduce_frames_device(self.handle, arr, n, ctypes.byref(params),
+                                                    u0_ptr, iters_ptr, status_ptr, cost_ptr,
+                                                    stream_ptr),
+              "ncs_hip_reduce_frames_device")
+
+
+_contexts = {}
+
+
+def context(device=0):
+    """Process-wide context of a device (created on first use)."""
+    with _lock:
+        ctx = _contexts.get(device)
+    if ctx is None:
+        ctx = Context(device)
+        with _lock:
+            _contexts[device] = ctx
+    return ctx

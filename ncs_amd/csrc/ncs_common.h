@@ -1,0 +1,55 @@
+// ncs_common.h -- shared device/host definitions of libncs_hip.so (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/ncs_hip.h"
+
+namespace ncs {
+
+constexpr int kTile = 16;          // tile edge (the reference's r_size = 16)
+constexpr int kTilePix = 256;      // pixels per tile
+constexpr int kWave = 64;          // gfx950 wavefront
+constexpr int kWavesPerBlock = 4;  // 256-thread workgroups, one tile per wave
+constexpr float kFitMin = 1.0e-6f; // negative-value guard (ncs.cl:45, ncs.c:169)
+
+// Per-lane OTF tables: 16 floats per lane (see ncs_solver.hip, MaskTables).
+constexpr int kTableFloats = kWave * 16;
+
+// One frame of a batch, as the kernel sees it.
+struct FrameDev {
+  const float *data;
+  const float *gamma;
+  float *out;
+  int H, W;
+  int ds, gs, os;      // row strides (elements)
+  float alpha;
+  int geom;            // NCS_GEOM_*
+  int ty;              // tiles along the fast axis
+  int pad;
+  long long first;     // global index of the frame's first tile
+};
+
+// Solver parameters in the kernel's units.
+struct SolveArgs {
+  const FrameDev *frames;
+  int n_frames;
+  long long total_tiles;
+  unsigned int *queue;         // device work-queue counter
+  const float *tables;         // [64][16] per-lane OTF/DFT tables
+  const float *u0;             // [tiles][256] warm start or nullptr
+  float *u_tiles;              // [tiles][256] solved tiles or nullptr
+  int *iterations;             // [tiles] or nullptr
+  int *status;                 // [tiles] or nullptr
+  float *cost;                 // [tiles] or nullptr
+  int *evaluations;            // [tiles] or nullptr
+  int max_it;
+  float eps;
+  int ls;
+  int max_ls;
+  float ftol, gtol, xtol, min_step, max_step, stepm;
+  int fallback;
+};
+
+}  // namespace ncs

@@ -1,0 +1,772 @@
+// ncs_solver.hip -- persistent per-tile L-BFGS solver of the NCS cost on gfx950.
+//
+// One wave64 owns one 16x16 tile at a time and keeps the whole solver state
+// (data, gamma, iterate, gradient, NC gradient, search direction and the
+// L-BFGS (s, y) history) in VGPRs: lane l = 16*g + j holds pixels
+// (4g + r, j), r = 0..3 -- the accumulator layout of
+// v_mfma_f32_16x16x4_f32.  Waves pull tiles from a device-side atomic work
+// queue, so tiles of every frame of a batch (any sizes, ROIs, per-frame
+// alpha) share one launch and finish at their own iteration counts.
+//
+// Cost per tile (clib/pyCNCS/ncs.c:335-364, opencl/ncs.cl:875-935):
+//   f(u) = sum_p [ lin_p u_p - c_p ln max(u_p + gamma_p, 1e-6) ] + alpha NC(u)
+//   NC(u) = (1/N^2) sum_kl q_kl |DFT2(u)_kl|^2,   q = mask^2 (FFT order)
+// with c = data + gamma and lin = 1 (lin = c = 0 on zero-padded pixels).
+//
+// NC on the matrix cores.  With R the orthonormal real DFT basis
+// (rows: cos k, k = 0..8, then sin k, k = 1..7) and Z = R X R^T,
+// NC = sum_ab qe(ab) Z_ab^2 + sum over (cc,ss)/(cs,sc) partner pairs
+// (the odd-odd part of q, zero for masks even in each axis), so
+//   grad NC = R^T W R,  W = 2 qe * Z + qo * partner(Z),
+// four 16x16x16 FP32 GEMMs = 16 MFMA 16x16x4 per application, every operand
+// taken straight from the register it sits in (no LDS, no shuffles; the
+// general-mask partner is one lane permutation, lane ^ 40).
+//
+// NC is quadratic, so along a search direction d:
+//   NC(u + t d) = NC(u) + t (gnc . d) + t^2/2 (d . K d),  gnc(u + t d) = gnc + t K d.
+// Each iteration applies K once (to d); line-search trials only re-evaluate
+// the likelihood, and the decrease f(u + t d) - f(u) is formed from log1p of
+// the relative change, which keeps the Armijo / Wolfe tests meaningful in
+// FP32 down to the gradient tolerance.
+#include "ncs_common.h"
+
+namespace ncs {
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// Wave primitives
+// ---------------------------------------------------------------------------
+
+template <int CTRL, int RM = 0xF, int BM = 0xF>
+__device__ __forceinline__ float dppmov(float v) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, RM, BM, false));
+}
+
+__device__ __forceinline__ float lane63(float x) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 63));
+}
+
+// Sum of N values over the 64 lanes; every lane receives the totals.
+// Row sums by DPP butterflies (quad_perm, half-mirror, mirror), rows combined
+// with row_bcast15 / row_bcast31 into lane 63, then broadcast by readlane.
+template <int N>
+__device__ __forceinline__ void wave_sum(float (&x)[N]) {
+#pragma unroll
+  for (int i = 0; i < N; i++) x[i] += dppmov<0xB1>(x[i]);
+#pragma unroll
+  for (int i = 0; i < N; i++) x[i] += dppmov<0x4E>(x[i]);
+#pragma unroll
+  for (int i = 0; i < N; i++) x[i] += dppmov<0x141>(x[i]);
+#pragma unroll
+  for (int i = 0; i < N; i++) x[i] += dppmov<0x140>(x[i]);
+#pragma unroll
+  for (int i = 0; i < N; i++) x[i] += dppmov<0x142, 0xA>(x[i]);
+#pragma unroll
+  for (int i = 0; i < N; i++) x[i] += dppmov<0x143, 0xC>(x[i]);
+#pragma unroll
+  for (int i = 0; i < N; i++) x[i] = lane63(x[i]);
+}
+
+__device__ __forceinline__ float wave_dot(const float (&a)[4], const float (&b)[4]) {
+  float s[1] = {a[0] * b[0] + a[1] * b[1] + (a[2] * b[2] + a[3] * b[3])};
+  wave_sum<1>(s);
+  return s[0];
+}
+
+__device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// log(1 + x) accurate to a few ulp: atanh series for |x| < 1/4, log otherwise.
+__device__ __forceinline__ float log1p_acc(float x) {
+  if (fabsf(x) < 0.25f) {
+    float z = x * rcpf(2.0f + x);
+    float z2 = z * z;
+    float p = 0.11111111f;
+    p = fmaf(p, z2, 0.14285715f);
+    p = fmaf(p, z2, 0.2f);
+    p = fmaf(p, z2, 0.33333334f);
+    p = fmaf(p, z2, 1.0f);
+    return 2.0f * z * p;
+  }
+  return __logf(1.0f + x);
+}
+
+// ---------------------------------------------------------------------------
+// NC gradient operator on the matrix cores
+// ---------------------------------------------------------------------------
+
+struct MaskTables {
+  float ka[4];  // R[lane%16][4g + r]
+  float kb[4];  // R[4g + r][lane%16]
+  float qe[4];  // 2 * even-even part of q at Z position (lane%16, 4g + r)
+  float qo[4];  // +-2 * odd-odd part, coupling to the partner lane ^ 40
+};
+
+// gnc = K x where K = Hessian of NC (x, gnc in the tile layout).
+template <bool GEN>
+__device__ __forceinline__ void apply_k(const float (&x)[4], const MaskTables &T,
+                                        float (&out)[4]) {
+  f4 z1 = {0.f, 0.f, 0.f, 0.f};
+  // (R X)^T = X^T R^T : data as the A operand
+  z1 = __builtin_amdgcn_mfma_f32_16x16x4f32(x[0], T.ka[0], z1, 0, 0, 0);
+  z1 = __builtin_amdgcn_mfma_f32_16x16x4f32(x[1], T.ka[1], z1, 0, 0, 0);
+  z1 = __builtin_amdgcn_mfma_f32_16x16x4f32(x[2], T.ka[2], z1, 0, 0, 0);
+  z1 = __builtin_amdgcn_mfma_f32_16x16x4f32(x[3], T.ka[3], z1, 0, 0, 0);
+  f4 z = {0.f, 0.f, 0.f, 0.f};
+  // Z^T = R (R X)^T : data as the B operand
+  z = __builtin_amdgcn_mfma_f32_16x16x4f32(T.ka[0], z1[0], z, 0, 0, 0);
+  z = __builtin_amdgcn_mfma_f32_16x16x4f32(T.ka[1], z1[1], z, 0, 0, 0);
+  z = __builtin_amdgcn_mfma_f32_16x16x4f32(T.ka[2], z1[2], z, 0, 0, 0);
+  z = __builtin_amdgcn_mfma_f32_16x16x4f32(T.ka[3], z1[3], z, 0, 0, 0);
+  float w[4];
+#pragma unroll
+  for (int r = 0; r < 4; r++) w[r] = T.qe[r] * z[r];
+  if constexpr (GEN) {
+#pragma unroll
+    for (int r = 0; r < 4; r++) w[r] = fmaf(T.qo[r], __shfl_xor(z[r], 40), w[r]);
+  }
+  f4 e = {0.f, 0.f, 0.f, 0.f};
+  // W R : W^T held, so data as the A operand
+  e = __builtin_amdgcn_mfma_f32_16x16x4f32(w[0], T.kb[0], e, 0, 0, 0);
+  e = __builtin_amdgcn_mfma_f32_16x16x4f32(w[1], T.kb[1], e, 0, 0, 0);
+  e = __builtin_amdgcn_mfma_f32_16x16x4f32(w[2], T.kb[2], e, 0, 0, 0);
+  e = __builtin_amdgcn_mfma_f32_16x16x4f32(w[3], T.kb[3], e, 0, 0, 0);
+  f4 g = {0.f, 0.f, 0.f, 0.f};
+  // R^T (W R)
+  g = __builtin_amdgcn_mfma_f32_16x16x4f32(T.kb[0], e[0], g, 0, 0, 0);
+  g = __builtin_amdgcn_mfma_f32_16x16x4f32(T.kb[1], e[1], g, 0, 0, 0);
+  g = __builtin_amdgcn_mfma_f32_16x16x4f32(T.kb[2], e[2], g, 0, 0, 0);
+  g = __builtin_amdgcn_mfma_f32_16x16x4f32(T.kb[3], e[3], g, 0, 0, 0);
+#pragma unroll
+  for (int r = 0; r < 4; r++) out[r] = g[r];
+}
+
+// ---------------------------------------------------------------------------
+// More-Thuente line search on uniform scalars (More & Thuente 1994; the
+// liblbfgs default, lbfgs.h:704-719).  phi(t) = f(u + t d) - f(u).
+// ---------------------------------------------------------------------------
+
+enum LsResult { LS_CONTINUE = 0, LS_ACCEPT = 1, LS_FAIL_PRECISION = -1,
+                LS_FAIL_MINSTEP = -2, LS_FAIL_INCREASE = -3 };
+
+struct MtState {
+  float stx, fx, dx, sty, fy, dy;
+  float stmin, stmax, width, prev_width;
+  float dginit, dgtest;
+  int count;
+  bool brackt, stage1, uinfo;
+};
+
+__device__ __forceinline__ float fmax3(float a, float b, float c) {
+  return fmaxf(fmaxf(a, b), c);
+}
+
+// Safeguarded step (section 4 of the paper); false on inconsistent input.
+__device__ bool mt_cstep(float &stx, float &fx, float &dx, float &sty, float &fy,
+                         float &dy, float &stp, float fp, float dp, bool &brackt,
+                         float stmin, float stmax) {
+  const float t = stp;
+  if (brackt) {
+    float lo = fminf(stx, sty), hi = fmaxf(stx, sty);
+    if (t <= lo || hi <= t) return false;
+    if (0.0f <= dx * (t - stx)) return false;
+    if (stmax < stmin) return false;
+  }
+  const float sgnd = dp * (dx / fabsf(dx));
+  float stpf, stpc, stpq, theta, s, gam, p, q;
+  bool bound;
+  if (fp > fx) {
+    bound = true;
+    theta = 3.0f * (fx - fp) / (t - stx) + dx + dp;
+    s = fmax3(fabsf(theta), fabsf(dx), fabsf(dp));
+    gam = s * sqrtf(fmaxf(0.0f, (theta / s) * (theta / s) - (dx / s) * (dp / s)));
+    if (t < stx) gam = -gam;
+    p = (gam - dx) + theta;
+    q = ((gam - dx) + gam) + dp;
+    stpc = stx + (p / q) * (t - stx);
+    stpq = stx + ((dx / ((fx - fp) / (t - stx) + dx)) / 2.0f) * (t - stx);
+    stpf = (fabsf(stpc - stx) < fabsf(stpq - stx)) ? stpc : stpc + (stpq - stpc) / 2.0f;
+    brackt = true;
+  } else if (sgnd < 0.0f) {
+    bound = false;
+    theta = 3.0f * (fx - fp) / (t - stx) + dx + dp;
+    s = fmax3(fabsf(theta), fabsf(dx), fabsf(dp));
+    gam = s * sqrtf(fmaxf(0.0f, (theta / s) * (theta / s) - (dx / s) * (dp / s)));
+    if (t > stx) gam = -gam;
+    p = (gam - dp) + theta;
+    q = ((gam - dp) + gam) + dx;
+    stpc = t + (p / q) * (stx - t);
+    stpq = t + (dp / (dp - dx)) * (stx - t);
+    stpf = (fabsf(stpc - t) > fabsf(stpq - t)) ? stpc : stpq;
+    brackt = true;
+  } else if (fabsf(dp) < fabsf(dx)) {
+    bound = true;
+    theta = 3.0f * (fx - fp) / (t - stx) + dx + dp;
+    s = fmax3(fabsf(theta), fabsf(dx), fabsf(dp));
+    gam = s * sqrtf(fmaxf(0.0f, (theta / s) * (theta / s) - (dx / s) * (dp / s)));
+    if (t > stx) gam = -gam;
+    p = (gam - dp) + theta;
+    q = (gam + (dx - dp)) + gam;
+    float rr = p / q;
+    if (rr < 0.0f && gam != 0.0f)
+      stpc = t + rr * (stx - t);
+    else
+      stpc = (t > stx) ? stmax : stmin;
+    stpq = t + (dp / (dp - dx)) * (stx - t);
+    if (brackt)
+      stpf = (fabsf(t - stpc) < fabsf(t - stpq)) ? stpc : stpq;
+    else
+      stpf = (fabsf(t - stpc) > fabsf(t - stpq)) ? stpc : stpq;
+  } else {
+    bound = false;
+    if (brackt) {
+      theta = 3.0f * (fp - fy) / (sty - t) + dy + dp;
+      s = fmax3(fabsf(theta), fabsf(dy), fabsf(dp));
+      gam = s * sqrtf(fmaxf(0.0f, (theta / s) * (theta / s) - (dy / s) * (dp / s)));
+      if (t > sty) gam = -gam;
+      p = (gam - dp) + theta;
+      q = ((gam - dp) + gam) + dy;
+      stpc = t + (p / q) * (sty - t);
+      stpf = stpc;
+    } else {
+      stpf = (t > stx) ? stmax : stmin;
+    }
+  }
+  if (fp > fx) {
+    sty = t; fy = fp; dy = dp;
+  } else {
+    if (sgnd < 0.0f) { sty = stx; fy = fx; dy = dx; }
+    stx = t; fx = fp; dx = dp;
+  }
+  stpf = fminf(stpf, stmax);
+  stpf = fmaxf(stpf, stmin);
+  if (brackt && bound) {
+    float mq = stx + 0.66f * (sty - stx);
+    stpf = (stx < sty) ? fminf(mq, stpf) : fmaxf(mq, stpf);
+  }
+  stp = stpf;
+  return true;
+}
+
+__device__ __forceinline__ void mt_init(MtState &S, float dginit, const SolveArgs &A) {
+  S.stx = S.sty = 0.0f;
+  S.fx = S.fy = 0.0f;
+  S.dx = S.dy = dginit;
+  S.dginit = dginit;
+  S.dgtest = A.ftol * dginit;
+  S.width = A.max_step - A.min_step;
+  S.prev_width = 2.0f * S.width;
+  S.count = 0;
+  S.brackt = false;
+  S.stage1 = true;
+  S.uinfo = false;
+}
+
+// Clamp / fall back before a trial; returns the step to evaluate.
+__device__ __forceinline__ float mt_pre(MtState &S, float stp, const SolveArgs &A) {
+  if (S.brackt) {
+    S.stmin = fminf(S.stx, S.sty);
+    S.stmax = fmaxf(S.stx, S.sty);
+  } else {
+    S.stmin = S.stx;
+    S.stmax = stp + 4.0f * (stp - S.stx);
+  }
+  stp = fmaxf(stp, A.min_step);
+  stp = fminf(stp, A.max_step);
+  if ((S.brackt && ((stp <= S.stmin || S.stmax <= stp) ||
+                    A.max_ls <= S.count + 1 || S.uinfo)) ||
+      (S.brackt && (S.stmax - S.stmin <= A.xtol * S.stmax)))
+    stp = S.stx;
+  return stp;
+}
+
+// Tests after a trial (f = phi(stp), dg = phi'(stp)); on LS_CONTINUE the
+// next trial step is left in stp.
+__device__ __forceinline__ int mt_post(MtState &S, float &stp, float f, float dg,
+                                       const SolveArgs &A) {
+  const float ftest1 = stp * S.dgtest;
+  S.count++;
+  if (S.brackt && ((stp <= S.stmin || S.stmax <= stp) || S.uinfo))
+    return LS_FAIL_PRECISION;
+  if (stp == A.max_step && f <= ftest1 && dg <= S.dgtest) return LS_FAIL_PRECISION;
+  if (stp == A.min_step && (ftest1 < f || S.dgtest <= dg)) return LS_FAIL_MINSTEP;
+  if (S.brackt && (S.stmax - S.stmin) <= A.xtol * S.stmax) return LS_FAIL_PRECISION;
+  if (A.max_ls <= S.count) return LS_FAIL_PRECISION;
+  if (f <= ftest1 && fabsf(dg) <= A.gtol * (-S.dginit)) return LS_ACCEPT;
+  if (S.stage1 && f <= ftest1 && fminf(A.ftol, A.gtol) * S.dginit <= dg) S.stage1 = false;
+  bool ok;
+  if (S.stage1 && ftest1 < f && f <= S.fx) {
+    float fm = f - stp * S.dgtest, fxm = S.fx - S.stx * S.dgtest,
+          fym = S.fy - S.sty * S.dgtest;
+    float dgm = dg - S.dgtest, dgxm = S.dx - S.dgtest, dgym = S.dy - S.dgtest;
+    ok = mt_cstep(S.stx, fxm, dgxm, S.sty, fym, dgym, stp, fm, dgm, S.brackt,
+                  S.stmin, S.stmax);
+    S.fx = fxm + S.stx * S.dgtest;
+    S.fy = fym + S.sty * S.dgtest;
+    S.dx = dgxm + S.dgtest;
+    S.dy = dgym + S.dgtest;
+  } else {
+    ok = mt_cstep(S.stx, S.fx, S.dx, S.sty, S.fy, S.dy, stp, f, dg, S.brackt,
+                  S.stmin, S.stmax);
+  }
+  S.uinfo = !ok;
+  if (S.brackt) {
+    if (0.66f * S.prev_width <= fabsf(S.sty - S.stx)) stp = S.stx + 0.5f * (S.sty - S.stx);
+    S.prev_width = S.width;
+    S.width = fabsf(S.sty - S.stx);
+  }
+  // non-finite step (NaN data): give up rather than loop
+  if (!(fabsf(stp) <= 3.0e38f)) return LS_FAIL_PRECISION;
+  return LS_CONTINUE;
+}
+
+// ---------------------------------------------------------------------------
+// The persistent solver kernel
+// ---------------------------------------------------------------------------
+
+struct TileGeom {
+  int bx, by;           // window origin in the frame (may be -1)
+  int r_lo, r_hi;       // owned local rows [r_lo, r_hi)
+  int c_lo, c_hi;       // owned local cols
+};
+
+__device__ __forceinline__ TileGeom tile_geom(const FrameDev &F, long long local) {
+  TileGeom G;
+  int a = (int)(local / F.ty), b = (int)(local % F.ty);
+  if (F.geom == NCS_GEOM_REFERENCE) {
+    const int lastx = F.H - 15, lasty = F.W - 15;
+    G.bx = min(-1 + 14 * a, lastx);
+    G.by = min(-1 + 14 * b, lasty);
+    G.r_lo = 1;
+    G.r_hi = (G.bx == lastx) ? 15 : min(15, lastx + 1 - G.bx);
+    G.c_lo = 1;
+    G.c_hi = (G.by == lasty) ? 15 : min(15, lasty + 1 - G.by);
+  } else {
+    G.bx = 16 * a;
+    G.by = 16 * b;
+    G.r_lo = 0;
+    G.r_hi = min(16, F.H - G.bx);
+    G.c_lo = 0;
+    G.c_hi = min(16, F.W - G.by);
+  }
+  return G;
+}
+
+// Per-lane likelihood value and gradient at u (exact logs).
+__device__ __forceinline__ float ll_lane(const float (&u)[4], const float (&gm)[4],
+                                         const float (&c)[4], const float (&lin)[4],
+                                         float (&gll)[4]) {
+  float s = 0.0f;
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    float v = fmaxf(u[r] + gm[r], kFitMin);
+    s += lin[r] * u[r] - c[r] * logf(v);
+    gll[r] = lin[r] - c[r] / v;
+  }
+  return s;
+}
+
+template <int M, bool GEN>
+__global__ __launch_bounds__(256) void solve_kernel(SolveArgs A) {
+  const int lane = threadIdx.x & 63;
+  const int col = lane & 15, grp = lane >> 4;
+
+  MaskTables T;
+  {
+    const float4 *tb = reinterpret_cast<const float4 *>(A.tables + lane * 16);
+    float4 t0 = tb[0], t1 = tb[1], t2 = tb[2], t3 = tb[3];
+    T.ka[0] = t0.x; T.ka[1] = t0.y; T.ka[2] = t0.z; T.ka[3] = t0.w;
+    T.kb[0] = t1.x; T.kb[1] = t1.y; T.kb[2] = t1.z; T.kb[3] = t1.w;
+    T.qe[0] = t2.x; T.qe[1] = t2.y; T.qe[2] = t2.z; T.qe[3] = t2.w;
+    T.qo[0] = t3.x; T.qo[1] = t3.y; T.qo[2] = t3.z; T.qo[3] = t3.w;
+  }
+
+  for (;;) {
+    unsigned int tq = 0;
+    if (lane == 0) tq = atomicAdd(A.queue, 1u);
+    const long long t = (long long)(unsigned int)__builtin_amdgcn_readfirstlane((int)tq);
+    if (t >= A.total_tiles) break;
+
+    // frame of tile t (uniform binary search over the frame table)
+    int lo = 0, hi = A.n_frames - 1;
+    while (lo < hi) {
+      int mid = (lo + hi + 1) >> 1;
+      if (A.frames[mid].first <= t) lo = mid; else hi = mid - 1;
+    }
+    const FrameDev F = A.frames[lo];
+    const TileGeom G = tile_geom(F, t - F.first);
+    const float alpha = F.alpha;
+
+    // gather (edge clamp for the reference geometry, zero pad for blocks)
+    float dd[4], gm[4], c[4], lin[4], u[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int row = 4 * grp + r;
+      if (F.geom == NCS_GEOM_REFERENCE) {
+        const int gr = min(max(G.bx + row, 0), F.H - 1);
+        const int gc = min(max(G.by + col, 0), F.W - 1);
+        dd[r] = F.data[(long long)gr * F.ds + gc];
+        gm[r] = F.gamma[(long long)gr * F.gs + gc];
+        lin[r] = 1.0f;
+      } else {
+        const bool ok = (row < G.r_hi) && (col < G.c_hi);
+        const int gr = ok ? G.bx + row : 0, gc = ok ? G.by + col : 0;
+        dd[r] = ok ? F.data[(long long)gr * F.ds + gc] : 0.0f;
+        gm[r] = ok ? F.gamma[(long long)gr * F.gs + gc] : 0.0f;
+        lin[r] = ok ? 1.0f : 0.0f;
+      }
+      c[r] = lin[r] * (dd[r] + gm[r]);
+      u[r] = dd[r];
+    }
+
+    // starting point, cost and gradient
+    float gnc[4], gll[4], g[4];
+    apply_k<GEN>(u, T, gnc);
+    float f;
+    {
+      float red[2];
+      red[0] = ll_lane(u, gm, c, lin, gll);
+      red[1] = u[0] * gnc[0] + u[1] * gnc[1] + (u[2] * gnc[2] + u[3] * gnc[3]);
+      if (A.u0 != nullptr) {
+        // warm start: keep it unless the plain start scores a lower cost
+        float w[4], wg[4], wll[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) w[r] = A.u0[t * kTilePix + (4 * grp + r) * kTile + col];
+        apply_k<GEN>(w, T, wg);
+        float red2[2];
+        red2[0] = ll_lane(w, gm, c, lin, wll);
+        red2[1] = w[0] * wg[0] + w[1] * wg[1] + (w[2] * wg[2] + w[3] * wg[3]);
+        float both[4] = {red[0], red[1], red2[0], red2[1]};
+        wave_sum<4>(both);
+        const float f_plain = both[0] + 0.5f * alpha * both[1];
+        const float f_warm = both[2] + 0.5f * alpha * both[3];
+        if (!A.fallback || f_warm < f_plain || !(f_plain == f_plain)) {
+#pragma unroll
+          for (int r = 0; r < 4; r++) { u[r] = w[r]; gnc[r] = wg[r]; gll[r] = wll[r]; }
+          f = f_warm;
+        } else {
+          f = f_plain;
+        }
+      } else {
+        float r2[2] = {red[0], red[1]};
+        wave_sum<2>(r2);
+        f = r2[0] + 0.5f * alpha * r2[1];
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) g[r] = fmaf(alpha, gnc[r], gll[r]);
+
+    int status = NCS_REACHED_MAXITERS, iters = 0, evals = 1;
+    bool gnc_exact = true;
+    float gg, uu;
+    {
+      float red[2] = {g[0] * g[0] + g[1] * g[1] + (g[2] * g[2] + g[3] * g[3]),
+                      u[0] * u[0] + u[1] * u[1] + (u[2] * u[2] + u[3] * u[3])};
+      wave_sum<2>(red);
+      gg = red[0];
+      uu = red[1];
+    }
+    bool done = sqrtf(gg) <= A.eps * fmaxf(1.0f, sqrtf(uu));
+    if (done) status = NCS_SUCCESS;
+
+    float S[M][4], Y[M][4], rho[M], al[M];
+    int nh = 0;
+    float dvec[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) dvec[r] = -g[r];
+    float step = rsqrtf(gg);
+    float ys_last = 1.0f, yy_last = 1.0f;
+
+    for (int k = 1; !done && k <= A.max_it; k++) {
+      // K d and the quadratic coefficients of NC along d
+      float kd[4];
+      apply_k<GEN>(dvec, T, kd);
+      float dg0, a1, a2;
+      {
+        float red[3] = {
+            g[0] * dvec[0] + g[1] * dvec[1] + (g[2] * dvec[2] + g[3] * dvec[3]),
+            gnc[0] * dvec[0] + gnc[1] * dvec[1] + (gnc[2] * dvec[2] + gnc[3] * dvec[3]),
+            kd[0] * dvec[0] + kd[1] * dvec[1] + (kd[2] * dvec[2] + kd[3] * dvec[3])};
+        wave_sum<3>(red);
+        dg0 = red[0];
+        a1 = red[1];
+        a2 = red[2];
+      }
+      iters = k;
+      if (!(dg0 <= 0.0f)) {  // not a descent direction (or NaN)
+        status = NCS_INCREASING_GRADIENT;
+        break;
+      }
+
+      float v0[4];
+      bool grd0[4];
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        float w0 = u[r] + gm[r];
+        grd0[r] = !(w0 >= kFitMin);
+        v0[r] = fmaxf(w0, kFitMin);
+      }
+
+      // line search
+      float ut[4], gt[4], phi = 0.0f;
+      int ls = LS_CONTINUE;
+      MtState S_;
+      if (A.ls == NCS_LS_MORE_THUENTE) mt_init(S_, dg0, A);
+      float tstep = step;
+      int trials = 0;
+      for (;;) {
+        if (A.ls == NCS_LS_MORE_THUENTE) tstep = mt_pre(S_, tstep, A);
+        float red[2] = {0.0f, 0.0f};
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const float td = tstep * dvec[r];
+          ut[r] = u[r] + td;
+          const float w = ut[r] + gm[r];
+          const float vt = fmaxf(w, kFitMin);
+          const float delta = (grd0[r] || !(w >= kFitMin)) ? vt - v0[r] : td;
+          const float lg = log1p_acc(delta * rcpf(v0[r]));
+          red[0] += lin[r] * td - c[r] * lg;
+          gt[r] = lin[r] - c[r] * rcpf(vt);
+          red[1] += gt[r] * dvec[r];
+        }
+        wave_sum<2>(red);
+        evals++;
+        trials++;
+        phi = red[0] + alpha * tstep * (a1 + 0.5f * tstep * a2);
+        const float dphi = red[1] + alpha * (a1 + tstep * a2);
+        if (A.ls == NCS_LS_MORE_THUENTE) {
+          ls = mt_post(S_, tstep, phi, dphi, A);
+          if (ls != LS_CONTINUE) break;
+        } else {
+          // Armijo backtracking (ncs.cl:979-1035)
+          if (phi <= A.ftol * dg0 * tstep) { ls = LS_ACCEPT; break; }
+          tstep *= A.stepm;
+          if (!(tstep >= A.min_step)) { ls = LS_FAIL_MINSTEP; break; }
+          if (trials >= A.max_ls) { ls = LS_FAIL_PRECISION; break; }
+        }
+      }
+      if (ls != LS_ACCEPT) {
+        status = (ls == LS_FAIL_MINSTEP) ? NCS_MINIMUM_STEP
+                 : (ls == LS_FAIL_INCREASE) ? NCS_INCREASING_GRADIENT
+                                            : NCS_REACHED_MAXPRECISION;
+        break;
+      }
+
+      // accept u + t d
+      float s[4], gnew[4], gncn[4], y[4];
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        s[r] = ut[r] - u[r];
+        gncn[r] = fmaf(tstep, kd[r], gnc[r]);
+      }
+      const bool moved = __any((s[0] != 0.0f) | (s[1] != 0.0f) | (s[2] != 0.0f) | (s[3] != 0.0f));
+      gnc_exact = false;
+      if ((k & 15) == 0) {  // periodic exact refresh of the NC gradient
+        apply_k<GEN>(ut, T, gncn);
+        gnc_exact = true;
+      }
+      f += phi;
+      float ys, yy;
+      for (int pass = 0; pass < 2; pass++) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          gnew[r] = fmaf(alpha, gncn[r], gt[r]);
+          y[r] = gnew[r] - g[r];
+        }
+        float red[4] = {
+            y[0] * s[0] + y[1] * s[1] + (y[2] * s[2] + y[3] * s[3]),
+            y[0] * y[0] + y[1] * y[1] + (y[2] * y[2] + y[3] * y[3]),
+            gnew[0] * gnew[0] + gnew[1] * gnew[1] + (gnew[2] * gnew[2] + gnew[3] * gnew[3]),
+            ut[0] * ut[0] + ut[1] * ut[1] + (ut[2] * ut[2] + ut[3] * ut[3])};
+        wave_sum<4>(red);
+        ys = red[0];
+        yy = red[1];
+        gg = red[2];
+        uu = red[3];
+        done = sqrtf(gg) <= A.eps * fmaxf(1.0f, sqrtf(uu));
+        if (!done || gnc_exact) break;
+        // confirm convergence with an exact NC gradient
+        apply_k<GEN>(ut, T, gncn);
+        gnc_exact = true;
+      }
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        u[r] = ut[r];
+        g[r] = gnew[r];
+        gnc[r] = gncn[r];
+      }
+      if (done) { status = NCS_SUCCESS; break; }
+      if (!moved) { status = NCS_REACHED_MAXPRECISION; break; }
+      if (k == A.max_it) { status = NCS_REACHED_MAXITERS; break; }
+
+      // history update (newest pair at index M-1)
+      if (ys > 0.0f) {
+#pragma unroll
+        for (int j = 0; j < M - 1; j++) {
+#pragma unroll
+          for (int r = 0; r < 4; r++) { S[j][r] = S[j + 1][r]; Y[j][r] = Y[j + 1][r]; }
+          rho[j] = rho[j + 1];
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++) { S[M - 1][r] = s[r]; Y[M - 1][r] = y[r]; }
+        rho[M - 1] = 1.0f / ys;
+        ys_last = ys;
+        yy_last = yy;
+        nh = min(nh + 1, M);
+      }
+
+      // two-loop recursion
+#pragma unroll
+      for (int r = 0; r < 4; r++) dvec[r] = -g[r];
+#pragma unroll
+      for (int j = M - 1; j >= 0; j--) {
+        if (j >= M - nh) {
+          al[j] = rho[j] * wave_dot(S[j], dvec);
+#pragma unroll
+          for (int r = 0; r < 4; r++) dvec[r] = fmaf(-al[j], Y[j][r], dvec[r]);
+        }
+      }
+      const float hscale = (nh > 0) ? ys_last / yy_last : rsqrtf(gg);
+#pragma unroll
+      for (int r = 0; r < 4; r++) dvec[r] *= hscale;
+#pragma unroll
+      for (int j = 0; j < M; j++) {
+        if (j >= M - nh) {
+          const float b = rho[j] * wave_dot(Y[j], dvec);
+#pragma unroll
+          for (int r = 0; r < 4; r++) dvec[r] = fmaf(al[j] - b, S[j][r], dvec[r]);
+        }
+      }
+      step = 1.0f;
+    }
+
+    // exact final cost at the returned point
+    if (!gnc_exact) apply_k<GEN>(u, T, gnc);
+    {
+      float red[2];
+      red[0] = ll_lane(u, gm, c, lin, gll);
+      red[1] = u[0] * gnc[0] + u[1] * gnc[1] + (u[2] * gnc[2] + u[3] * gnc[3]);
+      wave_sum<2>(red);
+      f = red[0] + 0.5f * alpha * red[1];
+    }
+
+    // stitch the owned interior and per-tile results
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int row = 4 * grp + r;
+      if (F.out != nullptr && row >= G.r_lo && row < G.r_hi && col >= G.c_lo &&
+          col < G.c_hi)
+        F.out[(long long)(G.bx + row) * F.os + (G.by + col)] = u[r];
+      if (A.u_tiles != nullptr) A.u_tiles[t * kTilePix + row * kTile + col] = u[r];
+    }
+    if (lane == 0) {
+      if (A.iterations) A.iterations[t] = iters;
+      if (A.status) A.status[t] = status;
+      if (A.cost) A.cost[t] = f;
+      if (A.evaluations) A.evaluations[t] = evals;
+    }
+  }
+}
+
+// Cost and gradient of given tiles (one wave per tile), the analogue of
+// ncsSREvaluate (clib/pyCNCS/ncs.c:335-364) for parity tests.
+template <bool GEN>
+__global__ __launch_bounds__(256) void eval_kernel(const float *data, const float *gamma,
+                                                   const float *u_in, const float *tables,
+                                                   float alpha, int n, float *cost,
+                                                   float *grad) {
+  const int lane = threadIdx.x & 63;
+  const int col = lane & 15, grp = lane >> 4;
+  const long long t = (long long)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  if (t >= n) return;  // whole waves leave together
+  MaskTables T;
+  for (int r = 0; r < 4; r++) {
+    T.ka[r] = tables[lane * 16 + r];
+    T.kb[r] = tables[lane * 16 + 4 + r];
+    T.qe[r] = tables[lane * 16 + 8 + r];
+    T.qo[r] = tables[lane * 16 + 12 + r];
+  }
+  float u[4], gm[4], c[4], lin[4], gnc[4], gll[4];
+  for (int r = 0; r < 4; r++) {
+    const long long i = t * kTilePix + (4 * grp + r) * kTile + col;
+    u[r] = u_in[i];
+    gm[r] = gamma[i];
+    c[r] = data[i] + gm[r];
+    lin[r] = 1.0f;
+  }
+  apply_k<GEN>(u, T, gnc);
+  float red[2];
+  red[0] = ll_lane(u, gm, c, lin, gll);
+  red[1] = u[0] * gnc[0] + u[1] * gnc[1] + (u[2] * gnc[2] + u[3] * gnc[3]);
+  wave_sum<2>(red);
+  if (lane == 0 && cost) cost[t] = red[0] + 0.5f * alpha * red[1];
+  if (grad)
+    for (int r = 0; r < 4; r++)
+      grad[t * kTilePix + (4 * grp + r) * kTile + col] = fmaf(alpha, gnc[r], gll[r]);
+}
+
+hipError_t launch_eval(bool general, const float *data, const float *gamma, const float *u,
+                       const float *tables, float alpha, int n, float *cost, float *grad,
+                       hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  unsigned int blocks = (unsigned int)((n + kWavesPerBlock - 1) / kWavesPerBlock);
+  if (general)
+    hipLaunchKernelGGL(eval_kernel<true>, dim3(blocks), dim3(kWave * kWavesPerBlock), 0, s,
+                       data, gamma, u, tables, alpha, n, cost, grad);
+  else
+    hipLaunchKernelGGL(eval_kernel<false>, dim3(blocks), dim3(kWave * kWavesPerBlock), 0, s,
+                       data, gamma, u, tables, alpha, n, cost, grad);
+  return hipGetLastError();
+}
+
+// Resets the work-queue counter (a kernel, not a memset, so that captured
+// graphs replay it correctly).
+__global__ void reset_queue_kernel(unsigned int *q) {
+  if (threadIdx.x == 0) *q = 0u;
+}
+
+// Instantiations: history 1..8, with and without the odd-mask coupling.
+#define NCS_INST(M)                                               \
+  template __global__ void solve_kernel<M, false>(SolveArgs);     \
+  template __global__ void solve_kernel<M, true>(SolveArgs);
+NCS_INST(1) NCS_INST(2) NCS_INST(3) NCS_INST(4)
+NCS_INST(5) NCS_INST(6) NCS_INST(7) NCS_INST(8)
+#undef NCS_INST
+
+typedef void (*SolveKernelFn)(SolveArgs);
+
+static SolveKernelFn solve_kernel_for(int m, bool general) {
+  switch (m) {
+#define NCS_CASE(M) \
+  case M: return general ? solve_kernel<M, true> : solve_kernel<M, false>;
+    NCS_CASE(1) NCS_CASE(2) NCS_CASE(3) NCS_CASE(4)
+    NCS_CASE(5) NCS_CASE(6) NCS_CASE(7) NCS_CASE(8)
+#undef NCS_CASE
+    default: return nullptr;
+  }
+}
+
+// Workgroups of 4 waves one CU holds at once for this instantiation (0 if
+// the history length is unsupported).
+int solve_blocks_per_cu(int m, bool general) {
+  SolveKernelFn fn = solve_kernel_for(m, general);
+  if (!fn) return 0;
+  int blocks = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &blocks, reinterpret_cast<const void *>(fn), kWave * kWavesPerBlock, 0) !=
+      hipSuccess)
+    return 0;
+  return blocks;
+}
+
+hipError_t launch_solve(int m, bool general, unsigned int blocks, hipStream_t s,
+                        const SolveArgs &a) {
+  SolveKernelFn fn = solve_kernel_for(m, general);
+  if (!fn) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(64), 0, s, a.queue);
+  hipLaunchKernelGGL(fn, dim3(blocks), dim3(kWave * kWavesPerBlock), 0, s, a);
+  return hipGetLastError();
+}
+
+}  // namespace ncs

@@ -1,0 +1,139 @@
+"""GPU parity of the whole-image drop-in (tiling + solve + stitch)."""
+import ctypes
+
+import numpy
+import pytest
+
+import oracle
+from ncs_amd import _lib, masks, ncs, sim
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(autouse=True)
+def _needs_gpu(gpu_ctx):
+    return gpu_ctx
+
+
+def test_reference_image_test_crops():
+    """opencl/pyOpenCLNCS/test/test_im.py:16-44: C vs GPU reduceNoise on
+    30 x 30 crops at 144 offsets, r = 16, alpha = 0.02, atol 0.02."""
+    _say("reference test_im crops")
+    rng = numpy.random.RandomState(1)
+    im_size = 30
+    gamma = rng.uniform(2.0, 4.0, (im_size, im_size))
+    image = rng.uniform(0.01, 10.0, (im_size, im_size))
+    otfmask = masks.random_mask(16, numpy.random.default_rng(1))
+    worst = 0.0
+    for ix in range(12):
+        for iy in range(12):
+            a = oracle.cReduceNoise(image[ix:, iy:], gamma[ix:, iy:],
+                                    numpy.fft.fftshift(otfmask), 0.02)
+            b = ncs.reduceNoise([image[ix:, iy:]], gamma[ix:, iy:], otfmask, 0.02)[0]
+            worst = max(worst, float(numpy.max(numpy.abs(a - b))))
+            assert numpy.allclose(a, b, atol=0.02), (ix, iy)
+    _say("worst abs diff over 144 crops", worst)
+
+
+@pytest.mark.parametrize("shape", [(14, 14), (15, 29), (19, 23), (30, 30), (128, 37), (200, 201)])
+def test_alpha_zero_is_bit_exact_identity(shape):
+    """alpha = 0 makes every tile's minimiser its data: gather, clamp and
+    stitch are then checked bit for bit."""
+    _say("identity", shape)
+    rng = numpy.random.default_rng(shape[0] * 1000 + shape[1])
+    image = rng.uniform(0.5, 100, shape).astype(numpy.float32)
+    gamma = rng.uniform(1, 10, shape).astype(numpy.float32)
+    out = ncs.reduceNoise([image], gamma, masks.otf_mask(), 0.0)[0]
+    assert numpy.array_equal(out, image)
+    out_b = ncs.reduceNoise([image], gamma, masks.otf_mask(), 0.0, geometry="blocks")[0]
+    assert numpy.array_equal(out_b, image)
+
+
+def test_parseval_closed_form_image(golden):
+    """All-ones mask: per-pixel closed form, whatever the tiling."""
+    _say("parseval image")
+    rng = numpy.random.default_rng(21)
+    d = rng.uniform(5, 60, (61, 77)).astype(numpy.float32)
+    g = rng.uniform(1, 20, (61, 77)).astype(numpy.float32)
+    a = 0.1
+    c = d.astype(numpy.float64) + g
+    expect = (-(1 + 2 * a * g) + numpy.sqrt((1 + 2 * a * g) ** 2 - 8 * a * (g - c))) / (4 * a)
+    p = _lib.default_params(epsilon=1e-6)
+    for geom in ("reference", "blocks"):
+        out = ncs.reduceNoise([d], g, numpy.ones((16, 16)), a, params=p, geometry=geom)[0]
+        err = numpy.max(numpy.abs(out - expect) / expect)
+        _say(geom, "parseval max rel err", err)
+        assert err < 1e-4
+
+
+def test_synthetic_frame_tile_costs_match_oracle():
+    """A 256 x 256 synthetic frame: per-tile final costs (FP64 re-evaluation
+    of the GPU tiles) within 1e-5 of the oracle's, stitched images close."""
+    _say("synthetic 256x256 frame")
+    fr = sim.make_frame(1, (256, 256))
+    mask = masks.otf_mask()
+    out, info = ncs.reduceNoise([fr["data"]], fr["gamma"], mask, 0.1, return_info=True)
+    ref, rinfo = oracle.reduce_noise(fr["data"].astype(numpy.float64),
+                                     fr["gamma"].astype(numpy.float64),
+                                     masks.to_fft_order(mask), 0.1)
+    assert len(info["status"]) == len(rinfo["status"]) == 361
+    assert numpy.all(info["status"] == 0)
+    rel = numpy.abs(info["cost"].astype(numpy.float64) - rinfo["cost"]) / numpy.abs(rinfo["cost"])
+    diff = numpy.abs(out[0] - ref)
+    _say("tile cost max rel", rel.max(), "pixel max abs diff", diff.max(),
+         "median", numpy.median(diff))
+    assert rel.max() <= 1e-5
+    assert diff.max() < 0.05 * max(1.0, numpy.max(ref))
+
+
+def test_uneven_batch_equals_separate_calls():
+    """Frames of different sizes, gammas and alphas in one launch give the same
+    pixels as one call per frame (each tile's solve is independent)."""
+    _say("uneven batch")
+    shapes = [(64, 64), (40, 97), (130, 33), (16, 16)]
+    rng = numpy.random.default_rng(31)
+    frames = [sim.make_frame(100 + i, s) for i, s in enumerate(shapes)]
+    alphas = [0.05, 0.1, 0.2, 0.3]
+    mask = masks.otf_mask()
+    batch = ncs.reduceNoise([f["data"] for f in frames], [f["gamma"] for f in frames],
+                            mask, alphas)
+    for f, a, b in zip(frames, alphas, batch):
+        one = ncs.reduceNoise([f["data"]], f["gamma"], mask, a)[0]
+        assert numpy.array_equal(one, b)
+    del rng
+
+
+def test_c_abi_drop_in_matches_python_drop_in():
+    """ncsReduceNoise with the clib signature (ncs.h:42) on the GPU."""
+    _say("C ABI drop-in")
+    fr = sim.make_frame(3, (50, 60))
+    mask = masks.otf_mask()
+    lib = _lib.load()
+    img = fr["data"].astype(numpy.float64)
+    gam = fr["gamma"].astype(numpy.float64)
+    out = numpy.zeros_like(img)
+    mf = numpy.ascontiguousarray(masks.to_fft_order(mask), dtype=numpy.float64)
+    lib.ncsReduceNoise(out.ctypes.data, img.ctypes.data, gam.ctypes.data, mf.ctypes.data,
+                       ctypes.c_double(0.1), 50, 60, 16)
+    py = ncs.reduceNoise([fr["data"]], fr["gamma"], mask, 0.1)[0]
+    assert numpy.array_equal(out.astype(numpy.float32), py)
+    c = ncs.cReduceNoise(img, gam, mf, 0.1)
+    assert numpy.array_equal(c, out)
+
+
+def test_reference_warnings_and_errors():
+    _say("warnings and errors")
+    fr = sim.make_frame(4, (32, 32))
+    with pytest.raises(ncs.NCSHipException):
+        ncs.reduceNoise([fr["data"]], fr["gamma"], numpy.ones((8, 8)), 0.1)
+    bad = numpy.random.default_rng(0).uniform(0, 1, (16, 16))
+    if not ncs.checkOTFMask(bad):
+        with pytest.raises(ncs.NCSHipException):
+            ncs.reduceNoise([fr["data"]], fr["gamma"], bad, 0.1)
+    with pytest.warns(UserWarning):
+        ncs.reduceNoise([fr["data"]], fr["gamma"], masks.otf_mask(), 0.1,
+                        params=_lib.default_params(max_iterations=1))
