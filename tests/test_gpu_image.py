@@ -62,12 +62,15 @@ def test_parseval_closed_form_image(golden):
     a = 0.1
     c = d.astype(numpy.float64) + g
     expect = (-(1 + 2 * a * g) + numpy.sqrt((1 + 2 * a * g) ** 2 - 8 * a * (g - c))) / (4 * a)
-    p = _lib.default_params(epsilon=1e-6)
+    # the stop rule |g| <= eps |u| with Hessian >= 2 alpha bounds the error by
+    # eps |u_tile| / (2 alpha) ~ 2.5e-4 (|u_tile| ~ 500) at eps = 1e-7
+    p = _lib.default_params(epsilon=1e-7)
     for geom in ("reference", "blocks"):
-        out = ncs.reduceNoise([d], g, numpy.ones((16, 16)), a, params=p, geometry=geom)[0]
-        err = numpy.max(numpy.abs(out - expect) / expect)
-        _say(geom, "parseval max rel err", err)
-        assert err < 1e-4
+        out, info = ncs.reduceNoise([d], g, numpy.ones((16, 16)), a, params=p, geometry=geom,
+                                    return_info=True)
+        err = numpy.max(numpy.abs(out[0] - expect))
+        _say(geom, "parseval max abs err", err, "status", numpy.unique(info["status"]))
+        assert err < 2.5e-4
 
 
 def test_synthetic_frame_tile_costs_match_oracle():
