@@ -117,8 +117,8 @@ int ncs_hip_reduce_frames(ncs_ctx *ctx, const ncs_frame *frames, int n_frames,
                           int *status, float *cost);
 
 /* Same with device pointers (frames[] itself in host memory), enqueued on
- * `stream` (a hipStream_t; NULL = the context's stream) without
- * synchronisation.  u0_tiles: device [tiles][256] for NCS_START_TILES, else
+ * `stream` (a hipStream_t; 0 is the default NULL stream, as torch's default
+ * stream) without waiting for completion.  u0_tiles: device [tiles][256] for NCS_START_TILES, else
  * NULL.  iterations/status/cost: device arrays or NULL. */
 int ncs_hip_reduce_frames_device(ncs_ctx *ctx, const ncs_frame *frames,
                                  int n_frames, const ncs_params *params,
@@ -157,6 +157,36 @@ int ncs_hip_eval_tiles(ncs_ctx *ctx, const float *data, const float *gamma,
 void ncsReduceNoise(double *ncs_image, double *image, double *gamma,
                     double *otf_mask, double alpha, int im_x, int im_y,
                     int r_size);
+
+/* ---- learned warm start: 4-layer 3x3 conv net on BF16 MFMA ---- */
+
+/* The net (no counterpart in the reference): 2 -> 32 -> 32 -> 32 -> 1
+ * channels, 3x3 kernels with zero padding inside each 16x16 tile, ReLU after
+ * layers 1-3.  Inputs per tile: mu = mean(d), sigma = sqrt(max(mu,1) +
+ * mean(gamma)), x0 = (d - mu)/sigma, x1 = gamma/sigma^2; the start is
+ * u0 = d + sigma * y.  Parameters are FP32, per layer the weights
+ * [cout][cin][3][3] then the bias (19,393 values); the kernels compute in
+ * BF16 with FP32 accumulation. */
+#define NCS_NET_PARAMS 19393
+
+/* Sets the context's network (host FP32 parameters) for NCS_START_LEARNED. */
+int ncs_hip_net_set_weights(ncs_ctx *ctx, const float *params_f32);
+
+/* Forward on device tensors: x [n][2][256] normalised inputs (FP32) ->
+ * out [n][256] (FP32).  acts (device, may be NULL) receives the post-ReLU
+ * activations of layers 1-3 as BF16 [3][n][256][32] for the backward pass.
+ * params: device FP32 parameters, or NULL for the context's weights.
+ * Enqueued on `stream`. */
+int ncs_hip_net_forward(ncs_ctx *ctx, const float *params, const float *x,
+                        float *out, void *acts, int n, void *stream);
+
+/* Backward: from dL/dout [n][256] (FP32) and the saved activations, writes
+ * dL/dparams (device FP32 [NCS_NET_PARAMS], overwritten).  grad_x must be
+ * NULL (the input gradient is not needed for training). */
+int ncs_hip_net_backward(ncs_ctx *ctx, const float *params, const float *x,
+                         const void *acts, const float *grad_out,
+                         float *grad_params, float *grad_x, int n,
+                         void *stream);
 
 #ifdef __cplusplus
 }

@@ -89,6 +89,9 @@ SIGNATURES = {
     "ncs_hip_solve_tiles_device": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _f, _P,
                                         _vp]),
     "ncs_hip_eval_tiles": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _i, _f]),
+    "ncs_hip_net_set_weights": (_i, [_vp, _vp]),
+    "ncs_hip_net_forward": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _vp]),
+    "ncs_hip_net_backward": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp]),
     "ncsReduceNoise": (None, [_vp, _vp, _vp, _vp, ctypes.c_double, _i, _i, _i]),
 }
 

@@ -13,9 +13,17 @@
 #include "ncs_common.h"
 
 namespace ncs {
-int solve_blocks_per_cu(int m, bool general);
-hipError_t launch_solve(int m, bool general, unsigned int blocks, hipStream_t s,
+int solve_waves_per_cu(int m, bool general, bool pad);
+hipError_t launch_solve(int m, bool general, bool pad, unsigned int waves, hipStream_t s,
                         const SolveArgs &a);
+hipError_t net_pack(const float *P, void *packs, hipStream_t s);
+size_t net_packs_bytes();
+hipError_t net_forward(const float *x, float *y, void *acts, const void *packs, const float *P,
+                       long long n, int grid, hipStream_t s);
+hipError_t net_backward(const float *x, const void *acts, const float *gout, float *gP,
+                        const void *packs, long long n, int grid, hipStream_t s);
+hipError_t net_warm(const FrameDev *frames, int n_frames, long long total, float *u0,
+                    const void *packs, const float *P, int grid, hipStream_t s);
 hipError_t launch_eval(bool general, const float *data, const float *gamma, const float *u,
                        const float *tables, float alpha, int n, float *cost, float *grad,
                        hipStream_t s);
@@ -83,8 +91,15 @@ struct ncs_ctx {
   std::map<int, int> tuned_waves;  // history -> waves per CU
   std::map<int, std::unique_ptr<GraphRec>> graphs;
   int next_graph = 1;
-  // conv net (learned warm start), device FP32 params
-  DevBuf net_params;
+  // last uploaded frame descriptors (pinned staging + completion event)
+  std::vector<FrameDev> last_frames;
+  bool frames_valid = false;
+  void *frames_staging = nullptr;
+  size_t frames_staging_cap = 0;
+  hipEvent_t frames_event = nullptr;
+  bool frames_event_pending = false;
+  // conv net (learned warm start): FP32 parameters and BF16 packs
+  DevBuf net_params, net_packs, net_tmp_packs, u0_buf;
   bool has_net = false;
 };
 
@@ -134,6 +149,7 @@ extern "C" int ncs_hip_create(int device, ncs_ctx **out) {
   c->cus = prop.multiProcessorCount;
   std::snprintf(c->arch, sizeof(c->arch), "%s", prop.gcnArchName);
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  HIPCHK(hipEventCreateWithFlags(&c->frames_event, hipEventDisableTiming));
   HIPCHK(c->queue.ensure(256));
   *out = c.release();
   return NCS_OK;
@@ -161,8 +177,10 @@ extern "C" void ncs_hip_destroy(ncs_ctx *ctx) {
   ctx->graphs.clear();
   DevBuf *bufs[] = {&ctx->tables, &ctx->frames, &ctx->queue, &ctx->data, &ctx->gamma,
                     &ctx->out, &ctx->iters, &ctx->status, &ctx->cost, &ctx->tiles_u,
-                    &ctx->net_params};
+                    &ctx->net_params, &ctx->net_packs, &ctx->net_tmp_packs, &ctx->u0_buf};
   for (DevBuf *b : bufs) b->release();
+  if (ctx->frames_event) (void)hipEventDestroy(ctx->frames_event);
+  if (ctx->frames_staging) (void)hipHostFree(ctx->frames_staging);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -308,18 +326,54 @@ static int build_frames(const ncs_frame *frames, int n, std::vector<FrameDev> &f
   return NCS_OK;
 }
 
-static int waves_for(ncs_ctx *ctx, const ncs_params *p) {
+// Frame descriptors go to the device through a pinned staging copy; a batch
+// identical to the previous one (the common streaming / benchmark case) is
+// not uploaded again.  The staging buffer is rewritten only after the event of
+// its last copy has completed.
+static int upload_frames(ncs_ctx *ctx, const std::vector<FrameDev> &fd, hipStream_t s) {
+  const size_t bytes = fd.size() * sizeof(FrameDev);
+  if (ctx->frames_valid && ctx->last_frames.size() == fd.size() &&
+      std::memcmp(ctx->last_frames.data(), fd.data(), bytes) == 0)
+    return NCS_OK;
+  if (ctx->frames_event_pending) {
+    HIPCHK(hipEventSynchronize(ctx->frames_event));
+    ctx->frames_event_pending = false;
+  }
+  if (bytes > ctx->frames_staging_cap) {
+    if (ctx->frames_staging) HIPCHK(hipHostFree(ctx->frames_staging));
+    ctx->frames_staging = nullptr;
+    ctx->frames_staging_cap = 0;
+    HIPCHK(hipHostMalloc(&ctx->frames_staging, bytes, hipHostMallocDefault));
+    ctx->frames_staging_cap = bytes;
+  }
+  HIPCHK(ctx->frames.ensure(bytes));
+  std::memcpy(ctx->frames_staging, fd.data(), bytes);
+  ctx->frames_valid = false;
+  HIPCHK(hipMemcpyAsync(ctx->frames.p, ctx->frames_staging, bytes, hipMemcpyHostToDevice, s));
+  HIPCHK(hipEventRecord(ctx->frames_event, s));
+  ctx->frames_event_pending = true;
+  ctx->last_frames = fd;
+  ctx->frames_valid = true;
+  return NCS_OK;
+}
+
+static bool any_pad(const ncs_frame *frames, int n) {
+  for (int i = 0; i < n; i++)
+    if (frames[i].geometry != NCS_GEOM_REFERENCE) return true;
+  return false;
+}
+
+static int waves_for(ncs_ctx *ctx, const ncs_params *p, bool pad) {
   if (p->waves_per_cu > 0) return p->waves_per_cu;
   auto it = ctx->tuned_waves.find(p->history);
   if (it != ctx->tuned_waves.end()) return it->second;
-  int blocks = solve_blocks_per_cu(p->history, ctx->general);
-  if (blocks <= 0) blocks = 2;
-  return blocks * kWavesPerBlock;
+  int w = solve_waves_per_cu(p->history, ctx->general, pad);
+  return w > 0 ? w : 8;
 }
 
 // Enqueue one persistent solve over device frames already described in
 // d_frames (device copy of fd).
-static int enqueue_solve(ncs_ctx *ctx, const FrameDev *d_frames, int n_frames,
+static int enqueue_solve(ncs_ctx *ctx, const FrameDev *d_frames, int n_frames, bool pad,
                          long long total, const ncs_params *p, const float *u0,
                          float *u_tiles, int *iters, int *status, float *cost,
                          unsigned int *queue, hipStream_t s) {
@@ -348,11 +402,9 @@ static int enqueue_solve(ncs_ctx *ctx, const FrameDev *d_frames, int n_frames,
   a.stepm = p->stepm;
   a.fallback = p->fallback;
   if (total <= 0) return NCS_OK;
-  int waves = waves_for(ctx, p);
-  long long blocks = ((long long)waves * ctx->cus + kWavesPerBlock - 1) / kWavesPerBlock;
-  long long need = (total + kWavesPerBlock - 1) / kWavesPerBlock;
-  blocks = std::max<long long>(1, std::min(blocks, need));
-  HIPCHK(launch_solve(p->history, ctx->general, (unsigned int)blocks, s, a));
+  long long waves = (long long)waves_for(ctx, p, pad) * ctx->cus;
+  waves = std::max<long long>(1, std::min(waves, total));
+  HIPCHK(launch_solve(p->history, ctx->general, pad, (unsigned int)waves, s, a));
   return NCS_OK;
 }
 
@@ -369,21 +421,26 @@ extern "C" int ncs_hip_reduce_frames_device(ncs_ctx *ctx, const ncs_frame *frame
   if (rc) return rc;
   if (params->start == NCS_START_TILES && !u0_tiles)
     return fail(NCS_E_ARG, "NCS_START_TILES needs u0_tiles");
-  if (params->start == NCS_START_LEARNED)
-    return fail(NCS_E_ARG, "the learned start runs through the Python warm-start layer");
+  if (params->start == NCS_START_LEARNED && !ctx->has_net)
+    return fail(NCS_E_NOWEIGHTS, "learned start requested but no network weights are set");
   std::vector<FrameDev> fd;
   long long total = 0;
   rc = build_frames(frames, n_frames, fd, total);
   if (rc) return rc;
   HIPCHK(set_device(ctx));
-  hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
-  HIPCHK(ctx->frames.ensure(fd.size() * sizeof(FrameDev)));
-  HIPCHK(hipMemcpyAsync(ctx->frames.p, fd.data(), fd.size() * sizeof(FrameDev),
-                        hipMemcpyHostToDevice, s));
-  // the descriptor copy must land before the host vector goes away
-  HIPCHK(hipStreamSynchronize(s));
-  return enqueue_solve(ctx, ctx->frames.as<FrameDev>(), n_frames, total, params,
-                       params->start == NCS_START_TILES ? u0_tiles : nullptr, nullptr,
+  hipStream_t s = (hipStream_t)stream;  // 0 = the default (NULL) stream
+  rc = upload_frames(ctx, fd, s);
+  if (rc) return rc;
+  const float *u0 = params->start == NCS_START_TILES ? u0_tiles : nullptr;
+  if (params->start == NCS_START_LEARNED) {
+    HIPCHK(ctx->u0_buf.ensure((size_t)total * kTilePix * sizeof(float)));
+    int grid = (int)std::min<long long>(total, (long long)ctx->cus * 4);
+    HIPCHK(net_warm(ctx->frames.as<FrameDev>(), n_frames, total, ctx->u0_buf.as<float>(),
+                    ctx->net_packs.p, ctx->net_params.as<float>(), grid, s));
+    u0 = ctx->u0_buf.as<float>();
+  }
+  return enqueue_solve(ctx, ctx->frames.as<FrameDev>(), n_frames, any_pad(frames, n_frames),
+                       total, params, u0, nullptr,
                        iterations, status, cost, ctx->queue.as<unsigned int>(), s);
 }
 
@@ -466,11 +523,10 @@ extern "C" int ncs_hip_reduce_frames(ncs_ctx *ctx, const ncs_frame *frames, int 
     fd[i].out = f.out ? dout + off[i] : nullptr;
     fd[i].ds = fd[i].gs = fd[i].os = f.width;
   }
-  HIPCHK(ctx->frames.ensure(fd.size() * sizeof(FrameDev)));
-  HIPCHK(hipMemcpyAsync(ctx->frames.p, fd.data(), fd.size() * sizeof(FrameDev),
-                        hipMemcpyHostToDevice, s));
-  rc = enqueue_solve(ctx, ctx->frames.as<FrameDev>(), n_frames, total, params, nullptr,
-                     nullptr, ctx->iters.as<int>(), ctx->status.as<int>(),
+  rc = upload_frames(ctx, fd, s);
+  if (rc) return rc;
+  rc = enqueue_solve(ctx, ctx->frames.as<FrameDev>(), n_frames, any_pad(frames, n_frames),
+                     total, params, nullptr, nullptr, ctx->iters.as<int>(), ctx->status.as<int>(),
                      ctx->cost.as<float>(), ctx->queue.as<unsigned int>(), s);
   if (rc) {
     (void)hipStreamSynchronize(s);
@@ -533,6 +589,73 @@ extern "C" int ncs_hip_eval_tiles(ncs_ctx *ctx, const float *data, const float *
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   b.release();
   HIPCHK(e);
+  return NCS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// learned warm start
+// ---------------------------------------------------------------------------
+
+extern "C" int ncs_hip_net_set_weights(ncs_ctx *ctx, const float *params_f32) {
+  if (!ctx || !params_f32) return fail(NCS_E_ARG, "bad arguments");
+  for (int i = 0; i < NCS_NET_PARAMS; i++)
+    if (!std::isfinite(params_f32[i])) return fail(NCS_E_ARG, "non-finite network parameter");
+  HIPCHK(set_device(ctx));
+  HIPCHK(ctx->net_params.ensure(NCS_NET_PARAMS * sizeof(float)));
+  HIPCHK(ctx->net_packs.ensure(net_packs_bytes()));
+  HIPCHK(hipMemcpyAsync(ctx->net_params.p, params_f32, NCS_NET_PARAMS * sizeof(float),
+                        hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(net_pack(ctx->net_params.as<float>(), ctx->net_packs.p, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  ctx->has_net = true;
+  return NCS_OK;
+}
+
+// Packs `params` (device) into the context's scratch packs, or returns the
+// context's own packs when params is NULL.
+static int packs_for(ncs_ctx *ctx, const float *params, hipStream_t s, const void **packs,
+                     const float **P) {
+  if (params) {
+    HIPCHK(ctx->net_tmp_packs.ensure(net_packs_bytes()));
+    HIPCHK(net_pack(params, ctx->net_tmp_packs.p, s));
+    *packs = ctx->net_tmp_packs.p;
+    *P = params;
+    return NCS_OK;
+  }
+  if (!ctx->has_net) return fail(NCS_E_NOWEIGHTS, "no network weights set");
+  *packs = ctx->net_packs.p;
+  *P = ctx->net_params.as<float>();
+  return NCS_OK;
+}
+
+extern "C" int ncs_hip_net_forward(ncs_ctx *ctx, const float *params, const float *x, float *out,
+                                   void *acts, int n, void *stream) {
+  if (!ctx || !x || !out || n < 0) return fail(NCS_E_ARG, "bad arguments");
+  HIPCHK(set_device(ctx));
+  hipStream_t s = (hipStream_t)stream;  // 0 = the default (NULL) stream
+  const void *packs;
+  const float *P;
+  int rc = packs_for(ctx, params, s, &packs, &P);
+  if (rc) return rc;
+  int grid = (int)std::max<long long>(1, std::min<long long>(n, (long long)ctx->cus * 4));
+  HIPCHK(net_forward(x, out, acts, packs, P, n, grid, s));
+  return NCS_OK;
+}
+
+extern "C" int ncs_hip_net_backward(ncs_ctx *ctx, const float *params, const float *x,
+                                    const void *acts, const float *grad_out, float *grad_params,
+                                    float *grad_x, int n, void *stream) {
+  if (!ctx || !x || !acts || !grad_out || !grad_params || n < 0)
+    return fail(NCS_E_ARG, "bad arguments");
+  if (grad_x) return fail(NCS_E_ARG, "the input gradient is not provided (pass NULL)");
+  HIPCHK(set_device(ctx));
+  hipStream_t s = (hipStream_t)stream;  // 0 = the default (NULL) stream
+  const void *packs;
+  const float *P;
+  int rc = packs_for(ctx, params, s, &packs, &P);
+  if (rc) return rc;
+  int grid = (int)std::max<long long>(1, std::min<long long>(n, (long long)ctx->cus * 4));
+  HIPCHK(net_backward(x, acts, grad_out, grad_params, packs, n, grid, s));
   return NCS_OK;
 }
 

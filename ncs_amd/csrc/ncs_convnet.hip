@@ -1,0 +1,546 @@
+// ncs_convnet.hip -- the learned warm start: a 4-layer 3x3 conv net
+// (2 -> 32 -> 32 -> 32 -> 1, ReLU after layers 1-3, 'same' zero padding
+// inside each 16x16 tile) on BF16 MFMA with FP32 accumulation.
+//
+// Input channels per tile (normalisation shared with ncs_amd/warmstart.py):
+//   mu = mean(d), sigma = sqrt(max(mu, 1) + mean(gamma)),
+//   x0 = (d - mu) / sigma,  x1 = gamma / sigma^2;
+// the net predicts y and the warm start is u0 = d + sigma * y.
+//
+// Convolution as GEMM on v_mfma_f32_16x16x32_bf16: for one output pixel row
+// y and a block of 16 output channels,
+//   D[cout][x] += sum_taps  W_tap[cout][cin] * In[cin][y + dy][x + dx],
+// A = weights (pre-packed per tap and block: lane (g, i) holds cout i,
+// cin 8g..8g+7), B = activations (lane (g, x) holds cin 8g..8g+7 of pixel
+// x), D: lane (g, x) holds couts 4g..4g+3 of pixel x.  Activations live in
+// LDS as [pixel][32] BF16, one 1 KB row per read; the dx = -1 / +1 taps are
+// DPP row_shr:1 / row_shl:1 of the same row, whose zero fill at the row
+// end is exactly the zero padding of the tile edge.  A 3-row sliding
+// window keeps every input row read once per layer.
+//
+// Backward: the data gradient is the same convolution with flipped,
+// transposed weight packs; the weight gradient is a GEMM over pixels
+// (MFMA with gathered operands), accumulated into FP32 with atomics.
+#include "ncs_common.h"
+
+namespace ncs {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef int i4 __attribute__((ext_vector_type(4)));
+
+constexpr int kCh = 32;
+// FP32 parameter vector: per layer weights [cout][cin][3][3] then bias
+constexpr int OFF_W1 = 0, OFF_B1 = 576, OFF_W2 = 608, OFF_B2 = 9824, OFF_W3 = 9856,
+              OFF_B3 = 19072, OFF_W4 = 19104, OFF_B4 = 19392, N_PARAMS = 19393;
+static_assert(N_PARAMS == NCS_NET_PARAMS, "parameter count");
+// A-operand packs (64 lanes x 8 BF16 = 1 KB each)
+constexpr int PK_L1 = 0;   // 2: layer 1, K = (tap, cin) padded to 32
+constexpr int PK_L2 = 2;   // 18: block * 9 + tap
+constexpr int PK_L3 = 20;  // 18
+constexpr int PK_L4 = 38;  // 9: tap (cout 0 in row 0)
+constexpr int PK_B4 = 47;  // 2: layer-4 data gradient, K = tap padded to 32
+constexpr int PK_B3 = 49;  // 18: layer-3 data gradient (flipped, transposed)
+constexpr int PK_B2 = 67;  // 18: layer-2 data gradient
+constexpr int N_PACKS = 85;
+
+__device__ __forceinline__ float w_at(const float *P, int off, int cin_n, int co, int ci,
+                                      int tap) {
+  return P[off + (co * cin_n + ci) * 9 + tap];
+}
+
+// Builds the BF16 A-operand packs from the FP32 parameters.
+__global__ void net_pack_kernel(const float *__restrict__ P, __bf16 *__restrict__ packs) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= N_PACKS * 512) return;
+  const int pack = idx >> 9, lane = (idx >> 3) & 63, e = idx & 7;
+  const int i = lane & 15, g = lane >> 4;
+  const int k = 8 * g + e;
+  float v = 0.0f;
+  if (pack < PK_L2) {  // layer 1: k = tap * 2 + cin
+    const int co = 16 * (pack - PK_L1) + i;
+    if (k < 18) v = w_at(P, OFF_W1, 2, co, k & 1, k >> 1);
+  } else if (pack < PK_L3) {
+    const int q = pack - PK_L2, co = 16 * (q / 9) + i;
+    v = w_at(P, OFF_W2, kCh, co, k, q % 9);
+  } else if (pack < PK_L4) {
+    const int q = pack - PK_L3, co = 16 * (q / 9) + i;
+    v = w_at(P, OFF_W3, kCh, co, k, q % 9);
+  } else if (pack < PK_B4) {
+    const int tap = pack - PK_L4;
+    if (i == 0) v = w_at(P, OFF_W4, kCh, 0, k, tap);
+  } else if (pack < PK_B3) {  // d(layer-3 act) from d(out): cout' = L4 cin, k = tap
+    const int co = 16 * (pack - PK_B4) + i;
+    if (k < 9) v = w_at(P, OFF_W4, kCh, 0, co, 8 - k);
+  } else if (pack < PK_B2) {
+    const int q = pack - PK_B3, co = 16 * (q / 9) + i;  // co = L3 cin, k = L3 cout
+    v = w_at(P, OFF_W3, kCh, k, co, 8 - q % 9);
+  } else {
+    const int q = pack - PK_B2, co = 16 * (q / 9) + i;
+    v = w_at(P, OFF_W2, kCh, k, co, 8 - q % 9);
+  }
+  packs[idx] = (__bf16)v;
+}
+
+__device__ __forceinline__ f4 mfma_bf16(i4 a, i4 b, f4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
+                                                 __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+}
+
+template <int CTRL>
+__device__ __forceinline__ i4 dpp_i4(i4 v) {
+  i4 r;
+  r.x = __builtin_amdgcn_update_dpp(0, v.x, CTRL, 0xF, 0xF, true);
+  r.y = __builtin_amdgcn_update_dpp(0, v.y, CTRL, 0xF, 0xF, true);
+  r.z = __builtin_amdgcn_update_dpp(0, v.z, CTRL, 0xF, 0xF, true);
+  r.w = __builtin_amdgcn_update_dpp(0, v.w, CTRL, 0xF, 0xF, true);
+  return r;
+}
+constexpr int kRowShr1 = 0x111;  // lane x <- lane x-1 in its row (0 at x = 0)
+constexpr int kRowShl1 = 0x101;  // lane x <- lane x+1 in its row (0 at x = 15)
+
+// Row y' of a [256][32] BF16 LDS activation buffer as a B operand
+// (lane (g, x): channels 8g..8g+7 of pixel (y', x)); zero outside the tile.
+__device__ __forceinline__ i4 load_row(const __bf16 *buf, int yy, int lane) {
+  i4 z = {0, 0, 0, 0};
+  if (yy < 0 || yy > 15) return z;
+  const int x = lane & 15, g = lane >> 4;
+  return *reinterpret_cast<const i4 *>(buf + ((yy * 16 + x) * kCh + 8 * g));
+}
+
+__device__ __forceinline__ void store4(__bf16 *buf, int y, int lane, int blk, f4 v) {
+  const int x = lane & 15, g = lane >> 4;
+  bf16x4 h;
+  h.x = (__bf16)v.x;
+  h.y = (__bf16)v.y;
+  h.z = (__bf16)v.z;
+  h.w = (__bf16)v.w;
+  *reinterpret_cast<bf16x4 *>(buf + ((y * 16 + x) * kCh + 16 * blk + 4 * g)) = h;
+}
+
+__device__ __forceinline__ f4 load4(const __bf16 *buf, int y, int lane, int blk) {
+  const int x = lane & 15, g = lane >> 4;
+  bf16x4 h = *reinterpret_cast<const bf16x4 *>(buf + ((y * 16 + x) * kCh + 16 * blk + 4 * g));
+  f4 r = {(float)h.x, (float)h.y, (float)h.z, (float)h.w};
+  return r;
+}
+
+// One 32 -> 16*NB channel layer over a tile: out rows via fn(y, blk, acc).
+// A operands: packs + (blk * 9 + tap); B: rows of `in`.
+template <int NB, class Epi>
+__device__ __forceinline__ void conv32(const __bf16 *in, const i4 *__restrict__ packs,
+                                       int lane, Epi epi) {
+  i4 A[NB][9];
+#pragma unroll
+  for (int b = 0; b < NB; b++)
+#pragma unroll
+    for (int t = 0; t < 9; t++) A[b][t] = packs[(b * 9 + t) * 64 + lane];
+  i4 rm = {0, 0, 0, 0};
+  i4 r0 = load_row(in, 0, lane);
+  for (int y = 0; y < 16; y++) {
+    const i4 rp = load_row(in, y + 1, lane);
+    f4 acc[NB];
+#pragma unroll
+    for (int b = 0; b < NB; b++) acc[b] = (f4){0.f, 0.f, 0.f, 0.f};
+    const i4 rows[3] = {rm, r0, rp};
+#pragma unroll
+    for (int dy = 0; dy < 3; dy++) {
+      const i4 c = rows[dy];
+      const i4 l = dpp_i4<kRowShr1>(c);
+      const i4 r = dpp_i4<kRowShl1>(c);
+#pragma unroll
+      for (int b = 0; b < NB; b++) {
+        acc[b] = mfma_bf16(A[b][dy * 3 + 0], l, acc[b]);
+        acc[b] = mfma_bf16(A[b][dy * 3 + 1], c, acc[b]);
+        acc[b] = mfma_bf16(A[b][dy * 3 + 2], r, acc[b]);
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < NB; b++) epi(y, b, acc[b]);
+    rm = r0;
+    r0 = rp;
+  }
+}
+
+// A layer whose input has NC <= 3 channels (K = tap * NC + c padded to 32),
+// input FP32 in LDS as [NC][256]; 2 output blocks.
+template <int NC, class Epi>
+__device__ __forceinline__ void conv_small(const float *in, const i4 *__restrict__ packs,
+                                           int lane, Epi epi) {
+  const i4 A0 = packs[lane], A1 = packs[64 + lane];
+  const int x = lane & 15, g = lane >> 4;
+  for (int y = 0; y < 16; y++) {
+    bf16x8 bv;
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      const int k = 8 * g + e;
+      float v = 0.0f;
+      if (k < 9 * NC) {
+        const int tap = k / NC, ch = k % NC;
+        const int yy = y + tap / 3 - 1, xx = x + tap % 3 - 1;
+        if (yy >= 0 && yy < 16 && xx >= 0 && xx < 16) v = in[ch * 256 + yy * 16 + xx];
+      }
+      bv[e] = (__bf16)v;
+    }
+    const i4 B = __builtin_bit_cast(i4, bv);
+    f4 z = {0.f, 0.f, 0.f, 0.f};
+    epi(y, 0, mfma_bf16(A0, B, z));
+    epi(y, 1, mfma_bf16(A1, B, z));
+  }
+}
+
+// Gathers one tile in the solver's reference tiling and builds the two
+// normalised input channels into in2 ([2][256] FP32 LDS); returns mu/sigma
+// and the tile data in the lane layout of the solver (rows 4g + r, col x).
+__device__ __forceinline__ void tile_inputs(const float *d_tile_lane, const float *gm_lane,
+                                            float *in2, int lane, float &sigma) {
+  float red[2] = {d_tile_lane[0] + d_tile_lane[1] + (d_tile_lane[2] + d_tile_lane[3]),
+                  gm_lane[0] + gm_lane[1] + (gm_lane[2] + gm_lane[3])};
+  for (int off = 32; off >= 1; off >>= 1) {
+    red[0] += __shfl_xor(red[0], off);
+    red[1] += __shfl_xor(red[1], off);
+  }
+  const float mu = red[0] * (1.0f / 256.0f), gmean = red[1] * (1.0f / 256.0f);
+  sigma = sqrtf(fmaxf(mu, 1.0f) + gmean);
+  const float inv = 1.0f / sigma;
+  const int x = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const int p = (4 * g + r) * 16 + x;
+    in2[p] = (d_tile_lane[r] - mu) * inv;
+    in2[256 + p] = gm_lane[r] * inv * inv;
+  }
+}
+
+// The network body on one tile whose inputs are in in2; the output y is left
+// in ybuf ([256] FP32, raster order).  acts (may be NULL): post-ReLU layer
+// 1-3 activations, BF16 [3][n][256][32], of tile t.
+__device__ __forceinline__ void net_body(const float *in2, __bf16 *bufA, __bf16 *bufB,
+                                         float *ybuf, const i4 *__restrict__ packs,
+                                         const float *__restrict__ P, int lane, __bf16 *acts,
+                                         long long n, long long t) {
+  const int x = lane & 15, g = lane >> 4;
+  // bias of the couts this lane writes: blk * 16 + 4g + r
+  float b1[2][4], b2[2][4], b3[2][4];
+#pragma unroll
+  for (int b = 0; b < 2; b++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      b1[b][r] = P[OFF_B1 + 16 * b + 4 * g + r];
+      b2[b][r] = P[OFF_B2 + 16 * b + 4 * g + r];
+      b3[b][r] = P[OFF_B3 + 16 * b + 4 * g + r];
+    }
+  auto relu_store = [&](__bf16 *dst, float (&bias)[2][4], int layer) {
+    return [=, &bias](int y, int b, f4 acc) {
+      f4 v;
+      v.x = fmaxf(acc.x + bias[b][0], 0.0f);
+      v.y = fmaxf(acc.y + bias[b][1], 0.0f);
+      v.z = fmaxf(acc.z + bias[b][2], 0.0f);
+      v.w = fmaxf(acc.w + bias[b][3], 0.0f);
+      store4(dst, y, lane, b, v);
+      if (acts != nullptr) {
+        __bf16 *a = acts + (((long long)layer * n + t) * 256) * kCh;
+        store4(a, y, lane, b, v);
+      }
+    };
+  };
+  // every phase reads what other lanes wrote in the previous one: the
+  // barriers order the LDS traffic (one wave per workgroup)
+  __syncthreads();
+  conv_small<2>(in2, packs + PK_L1 * 64, lane, relu_store(bufA, b1, 0));
+  __syncthreads();
+  conv32<2>(bufA, packs + PK_L2 * 64, lane, relu_store(bufB, b2, 1));
+  __syncthreads();
+  conv32<2>(bufB, packs + PK_L3 * 64, lane, relu_store(bufA, b3, 2));
+  __syncthreads();
+  const float b4 = P[OFF_B4];
+  conv32<1>(bufA, packs + PK_L4 * 64, lane, [&](int y, int, f4 acc) {
+    if (g == 0) ybuf[y * 16 + x] = acc.x + b4;
+  });
+  __syncthreads();
+}
+
+constexpr int kNetLdsBytes = 2 * 256 * kCh * 2 + 3 * 256 * 4;  // bufA, bufB, in2, ybuf
+
+// Training / tensor mode: x [n][2][256] normalised inputs -> y [n][256].
+__global__ __launch_bounds__(64) void net_fwd_kernel(const float *__restrict__ xin,
+                                                     float *__restrict__ yout,
+                                                     __bf16 *__restrict__ acts,
+                                                     const i4 *__restrict__ packs,
+                                                     const float *__restrict__ P, long long n) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  __bf16 *bufA = reinterpret_cast<__bf16 *>(lds_raw);
+  __bf16 *bufB = bufA + 256 * kCh;
+  float *in2 = reinterpret_cast<float *>(bufB + 256 * kCh);
+  float *ybuf = in2 + 512;
+  const int lane = threadIdx.x;
+  for (long long t = blockIdx.x; t < n; t += gridDim.x) {
+    for (int i = lane; i < 512; i += 64) in2[i] = xin[t * 512 + i];
+    net_body(in2, bufA, bufB, ybuf, packs, P, lane, acts, n, t);
+    for (int i = lane; i < 256; i += 64) yout[t * 256 + i] = ybuf[i];
+    __syncthreads();
+  }
+}
+
+// Inference mode over frames: gather each window exactly like the solver,
+// normalise, run the net and write u0 = d + sigma * y as [tiles][256].
+__global__ __launch_bounds__(64) void net_warm_kernel(const FrameDev *__restrict__ frames,
+                                                      int n_frames, long long total,
+                                                      float *__restrict__ u0,
+                                                      const i4 *__restrict__ packs,
+                                                      const float *__restrict__ P) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  __bf16 *bufA = reinterpret_cast<__bf16 *>(lds_raw);
+  __bf16 *bufB = bufA + 256 * kCh;
+  float *in2 = reinterpret_cast<float *>(bufB + 256 * kCh);
+  float *ybuf = in2 + 512;
+  const int lane = threadIdx.x;
+  const int col = lane & 15, grp = lane >> 4;
+  for (long long t = blockIdx.x; t < total; t += gridDim.x) {
+    int lo = 0, hi = n_frames - 1;
+    while (lo < hi) {
+      int mid = (lo + hi + 1) >> 1;
+      if (frames[mid].first <= t) lo = mid; else hi = mid - 1;
+    }
+    const FrameDev F = frames[lo];
+    const long long local = t - F.first;
+    const int a = (int)(local / F.ty), b = (int)(local % F.ty);
+    int bx, by, hv = 16, wv = 16;
+    if (F.geom == NCS_GEOM_REFERENCE) {
+      bx = min(-1 + 14 * a, F.H - 15);
+      by = min(-1 + 14 * b, F.W - 15);
+    } else {
+      bx = 16 * a;
+      by = 16 * b;
+      hv = min(16, F.H - bx);
+      wv = min(16, F.W - by);
+    }
+    float d[4], gm[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int row = 4 * grp + r;
+      if (F.geom == NCS_GEOM_REFERENCE) {
+        const int gr = min(max(bx + row, 0), F.H - 1), gc = min(max(by + col, 0), F.W - 1);
+        d[r] = F.data[(long long)gr * F.ds + gc];
+        gm[r] = F.gamma[(long long)gr * F.gs + gc];
+      } else {
+        const bool ok = row < hv && col < wv;
+        d[r] = ok ? F.data[(long long)(bx + row) * F.ds + by + col] : 0.0f;
+        gm[r] = ok ? F.gamma[(long long)(bx + row) * F.gs + by + col] : 0.0f;
+      }
+    }
+    float sigma;
+    tile_inputs(d, gm, in2, lane, sigma);
+    net_body(in2, bufA, bufB, ybuf, packs, P, lane, nullptr, 0, 0);
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int p = (4 * grp + r) * 16 + col;
+      u0[t * 256 + p] = d[r] + sigma * ybuf[p];
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// backward
+// ---------------------------------------------------------------------------
+
+// dW[co][ci][tap] += sum_p dz[p][co] * act[p + tap][ci] for 32-channel dz / act
+// ([256][32] BF16 LDS), MFMA over pixel chunks of 32 with gathered operands.
+__device__ __forceinline__ void grad_w32(const __bf16 *dz, const __bf16 *act, float *gW,
+                                         int lane) {
+  const int i = lane & 15, g = lane >> 4;
+  for (int tap = 0; tap < 9; tap++) {
+    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+#pragma unroll
+    for (int bo = 0; bo < 2; bo++)
+#pragma unroll
+      for (int bi = 0; bi < 2; bi++) {
+        f4 acc = {0.f, 0.f, 0.f, 0.f};
+        for (int ch = 0; ch < 8; ch++) {  // pixel chunks of 32
+          bf16x8 av, bv;
+#pragma unroll
+          for (int e = 0; e < 8; e++) {
+            const int p = ch * 32 + 8 * g + e;
+            av[e] = dz[p * kCh + 16 * bo + i];
+            const int yy = (p >> 4) + dy, xx = (p & 15) + dx;
+            bv[e] = (yy >= 0 && yy < 16 && xx >= 0 && xx < 16)
+                        ? act[(yy * 16 + xx) * kCh + 16 * bi + i]
+                        : (__bf16)0.0f;
+          }
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc, 0, 0, 0);
+        }
+        // acc: lane (g, i) holds co = 16 bo + 4g + r, ci = 16 bi + i
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+          atomicAdd(&gW[((16 * bo + 4 * g + r) * kCh + 16 * bi + i) * 9 + tap], acc[r]);
+      }
+  }
+}
+
+// d(bias)[co] += sum_p dz[p][co]
+__device__ __forceinline__ void grad_b32(const __bf16 *dz, float *gb, int lane) {
+  if (lane < kCh) {
+    float s = 0.0f;
+    for (int p = 0; p < 256; p++) s += (float)dz[p * kCh + lane];
+    atomicAdd(&gb[lane], s);
+  }
+}
+
+// Data gradient of a 32-channel layer output given via packs (flipped,
+// transposed weights), masked by the ReLU of `mask_act` and written in place
+// of it (each lane rewrites exactly the entries it read).
+__device__ __forceinline__ void grad_data32(const __bf16 *dz_next, __bf16 *act_inout,
+                                            const i4 *__restrict__ packs, int lane) {
+  conv32<2>(dz_next, packs, lane, [&](int y, int b, f4 acc) {
+    f4 a = load4(act_inout, y, lane, b);
+    f4 v;
+    v.x = a.x > 0.0f ? acc.x : 0.0f;
+    v.y = a.y > 0.0f ? acc.y : 0.0f;
+    v.z = a.z > 0.0f ? acc.z : 0.0f;
+    v.w = a.w > 0.0f ? acc.w : 0.0f;
+    store4(act_inout, y, lane, b, v);
+  });
+}
+
+__global__ __launch_bounds__(64) void net_bwd_kernel(const float *__restrict__ xin,
+                                                     const __bf16 *__restrict__ acts,
+                                                     const float *__restrict__ gout,
+                                                     float *__restrict__ gP,
+                                                     const i4 *__restrict__ packs,
+                                                     long long n) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  __bf16 *buf0 = reinterpret_cast<__bf16 *>(lds_raw);
+  __bf16 *buf1 = buf0 + 256 * kCh;
+  float *in2 = reinterpret_cast<float *>(buf1 + 256 * kCh);  // x, then dY
+  float *dy4 = in2 + 512;
+  const int lane = threadIdx.x;
+  const int i = lane & 15, g = lane >> 4;
+  auto load_act = [&](__bf16 *dst, int layer, long long t) {
+    const i4 *src = reinterpret_cast<const i4 *>(acts + (((long long)layer * n + t) * 256) * kCh);
+    i4 *d = reinterpret_cast<i4 *>(dst);
+    for (int k = lane; k < 256 * kCh / 8; k += 64) d[k] = src[k];
+  };
+  for (long long t = blockIdx.x; t < n; t += gridDim.x) {
+    for (int k = lane; k < 256; k += 64) dy4[k] = gout[t * 256 + k];
+    load_act(buf0, 2, t);  // layer-3 activations
+    __syncthreads();
+    // layer 4: dW4[0][ci][tap] = sum_p dy4[p] a3[p + tap][ci]; db4 = sum dy4
+    for (int q = lane; q < 9 * kCh; q += 64) {
+      const int ci = q % kCh, tap = q / kCh, dy = tap / 3 - 1, dx = tap % 3 - 1;
+      float s = 0.0f;
+      for (int p = 0; p < 256; p++) {
+        const int yy = (p >> 4) + dy, xx = (p & 15) + dx;
+        if (yy >= 0 && yy < 16 && xx >= 0 && xx < 16)
+          s += dy4[p] * (float)buf0[(yy * 16 + xx) * kCh + ci];
+      }
+      atomicAdd(&gP[OFF_W4 + ci * 9 + tap], s);
+    }
+    if (lane == 0) {
+      float s = 0.0f;
+      for (int p = 0; p < 256; p++) s += dy4[p];
+      atomicAdd(&gP[OFF_B4], s);
+    }
+    // dz3 = conv(dy4, flipped W4) masked by a3 > 0 -> buf1
+    conv_small<1>(dy4, packs + PK_B4 * 64, lane, [&](int y, int b, f4 acc) {
+      f4 a = load4(buf0, y, lane, b);
+      f4 v;
+      v.x = a.x > 0.0f ? acc.x : 0.0f;
+      v.y = a.y > 0.0f ? acc.y : 0.0f;
+      v.z = a.z > 0.0f ? acc.z : 0.0f;
+      v.w = a.w > 0.0f ? acc.w : 0.0f;
+      store4(buf1, y, lane, b, v);
+    });
+    __syncthreads();
+    load_act(buf0, 1, t);  // layer-2 activations
+    __syncthreads();
+    grad_w32(buf1, buf0, gP + OFF_W3, lane);
+    grad_b32(buf1, gP + OFF_B3, lane);
+    __syncthreads();
+    grad_data32(buf1, buf0, packs + PK_B3 * 64, lane);  // buf0 = dz2
+    __syncthreads();
+    load_act(buf1, 0, t);  // layer-1 activations
+    __syncthreads();
+    grad_w32(buf0, buf1, gP + OFF_W2, lane);
+    grad_b32(buf0, gP + OFF_B2, lane);
+    __syncthreads();
+    grad_data32(buf0, buf1, packs + PK_B2 * 64, lane);  // buf1 = dz1
+    // layer 1: dW1[co][ci][tap] = sum_p dz1[p][co] x[ci][p + tap]; K = pixels
+    for (int k = lane; k < 512; k += 64) in2[k] = xin[t * 512 + k];
+    __syncthreads();
+    for (int tap = 0; tap < 9; tap++) {
+      const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+#pragma unroll
+      for (int bo = 0; bo < 2; bo++) {
+        f4 acc = {0.f, 0.f, 0.f, 0.f};
+        for (int ch = 0; ch < 8; ch++) {
+          bf16x8 av, bv;
+#pragma unroll
+          for (int e = 0; e < 8; e++) {
+            const int p = ch * 32 + 8 * g + e;
+            av[e] = buf1[p * kCh + 16 * bo + i];
+            const int yy = (p >> 4) + dy, xx = (p & 15) + dx;
+            bv[e] = (i < 2 && yy >= 0 && yy < 16 && xx >= 0 && xx < 16)
+                        ? (__bf16)in2[i * 256 + yy * 16 + xx]
+                        : (__bf16)0.0f;
+          }
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc, 0, 0, 0);
+        }
+        if (i < 2)
+#pragma unroll
+          for (int r = 0; r < 4; r++)
+            atomicAdd(&gP[OFF_W1 + ((16 * bo + 4 * g + r) * 2 + i) * 9 + tap], acc[r]);
+      }
+    }
+    grad_b32(buf1, gP + OFF_B1, lane);
+    __syncthreads();
+  }
+}
+
+__global__ void fill_kernel(float *p, long long n, float v) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// launches
+// ---------------------------------------------------------------------------
+
+hipError_t net_pack(const float *P, void *packs, hipStream_t s) {
+  const int total = N_PACKS * 512;
+  hipLaunchKernelGGL(net_pack_kernel, dim3((total + 255) / 256), dim3(256), 0, s, P,
+                     reinterpret_cast<__bf16 *>(packs));
+  return hipGetLastError();
+}
+
+size_t net_packs_bytes() { return (size_t)N_PACKS * 512 * 2; }
+
+hipError_t net_forward(const float *x, float *y, void *acts, const void *packs, const float *P,
+                       long long n, int grid, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(net_fwd_kernel, dim3(grid), dim3(64), kNetLdsBytes, s, x, y,
+                     reinterpret_cast<__bf16 *>(acts), reinterpret_cast<const i4 *>(packs), P, n);
+  return hipGetLastError();
+}
+
+hipError_t net_backward(const float *x, const void *acts, const float *gout, float *gP,
+                        const void *packs, long long n, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(fill_kernel, dim3(64), dim3(256), 0, s, gP, (long long)N_PARAMS, 0.0f);
+  if (n <= 0) return hipGetLastError();
+  hipLaunchKernelGGL(net_bwd_kernel, dim3(grid), dim3(64), kNetLdsBytes, s, x,
+                     reinterpret_cast<const __bf16 *>(acts), gout, gP,
+                     reinterpret_cast<const i4 *>(packs), n);
+  return hipGetLastError();
+}
+
+hipError_t net_warm(const FrameDev *frames, int n_frames, long long total, float *u0,
+                    const void *packs, const float *P, int grid, hipStream_t s) {
+  if (total <= 0) return hipSuccess;
+  hipLaunchKernelGGL(net_warm_kernel, dim3(grid), dim3(64), kNetLdsBytes, s, frames, n_frames,
+                     total, u0, reinterpret_cast<const i4 *>(packs), P);
+  return hipGetLastError();
+}
+
+}  // namespace ncs

@@ -367,19 +367,60 @@ __device__ __forceinline__ float ll_lane(const float (&u)[4], const float (&gm)[
   return s;
 }
 
-template <int M, bool GEN>
-__global__ __launch_bounds__(256) void solve_kernel(SolveArgs A) {
-  const int lane = threadIdx.x & 63;
+// One trial of the line search at step t: the likelihood part of
+// phi(t) = f(u + t d) - f(u) and of phi'(t), and the likelihood gradient at
+// u + t d.  The NC part is the quadratic t a1 + t^2 a2 / 2 (added by the
+// caller).
+template <bool PAD>
+__device__ __forceinline__ void ll_trial(float t, const float (&u)[4], const float (&dv)[4],
+                                         const float (&gm)[4], const float (&c)[4],
+                                         const float (&lin)[4], float (&gt)[4],
+                                         float (&red)[2]) {
+  red[0] = 0.0f;
+  red[1] = 0.0f;
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const float td = t * dv[r];
+    const float w0 = u[r] + gm[r];
+    const float w = w0 + td;
+    const float v0 = fmaxf(w0, kFitMin);
+    const float vt = fmaxf(w, kFitMin);
+    // relative change of max(u + gamma, FITMIN); exact step t d when no guard
+    const float delta = (w0 >= kFitMin && w >= kFitMin) ? td : vt - v0;
+    const float lg = log1p_acc(delta * rcpf(v0));
+    const float l = PAD ? lin[r] : 1.0f;
+    red[0] += l * td - c[r] * lg;
+    gt[r] = l - c[r] * rcpf(vt);
+    red[1] += gt[r] * dv[r];
+  }
+  wave_sum<2>(red);
+}
+
+// One wave per workgroup; the wave's L-BFGS history lives in LDS as two rings
+// of M vectors (each lane keeps its own 4 values at float4 index
+// slot * 64 + lane, so no lane reads another lane's LDS and no barrier is
+// needed).  PAD enables zero-padded pixels (block geometry).
+template <int M, bool GEN, bool PAD>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8)))
+void solve_kernel(SolveArgs A) {
+  extern __shared__ float4 lds_hist[];
+  __shared__ MtState mt_lds;              // line-search state of this wave
+  float4 *const Sb = lds_hist;            // s vectors, ring of M
+  float4 *const Yb = lds_hist + M * 64;   // y vectors, ring of M
+  const int lane = threadIdx.x;
   const int col = lane & 15, grp = lane >> 4;
 
   MaskTables T;
   {
     const float4 *tb = reinterpret_cast<const float4 *>(A.tables + lane * 16);
-    float4 t0 = tb[0], t1 = tb[1], t2 = tb[2], t3 = tb[3];
+    float4 t0 = tb[0], t1 = tb[1], t2 = tb[2];
     T.ka[0] = t0.x; T.ka[1] = t0.y; T.ka[2] = t0.z; T.ka[3] = t0.w;
     T.kb[0] = t1.x; T.kb[1] = t1.y; T.kb[2] = t1.z; T.kb[3] = t1.w;
     T.qe[0] = t2.x; T.qe[1] = t2.y; T.qe[2] = t2.z; T.qe[3] = t2.w;
-    T.qo[0] = t3.x; T.qo[1] = t3.y; T.qo[2] = t3.z; T.qo[3] = t3.w;
+    if constexpr (GEN) {
+      float4 t3 = tb[3];
+      T.qo[0] = t3.x; T.qo[1] = t3.y; T.qo[2] = t3.z; T.qo[3] = t3.w;
+    }
   }
 
   for (;;) {
@@ -399,25 +440,26 @@ __global__ __launch_bounds__(256) void solve_kernel(SolveArgs A) {
     const float alpha = F.alpha;
 
     // gather (edge clamp for the reference geometry, zero pad for blocks)
-    float dd[4], gm[4], c[4], lin[4], u[4];
+    float gm[4], c[4], lin[4], u[4];
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       const int row = 4 * grp + r;
-      if (F.geom == NCS_GEOM_REFERENCE) {
+      float dd;
+      if (!PAD || F.geom == NCS_GEOM_REFERENCE) {
         const int gr = min(max(G.bx + row, 0), F.H - 1);
         const int gc = min(max(G.by + col, 0), F.W - 1);
-        dd[r] = F.data[(long long)gr * F.ds + gc];
+        dd = F.data[(long long)gr * F.ds + gc];
         gm[r] = F.gamma[(long long)gr * F.gs + gc];
         lin[r] = 1.0f;
       } else {
         const bool ok = (row < G.r_hi) && (col < G.c_hi);
         const int gr = ok ? G.bx + row : 0, gc = ok ? G.by + col : 0;
-        dd[r] = ok ? F.data[(long long)gr * F.ds + gc] : 0.0f;
+        dd = ok ? F.data[(long long)gr * F.ds + gc] : 0.0f;
         gm[r] = ok ? F.gamma[(long long)gr * F.gs + gc] : 0.0f;
         lin[r] = ok ? 1.0f : 0.0f;
       }
-      c[r] = lin[r] * (dd[r] + gm[r]);
-      u[r] = dd[r];
+      c[r] = lin[r] * (dd + gm[r]);
+      u[r] = dd;
     }
 
     // starting point, cost and gradient
@@ -434,10 +476,11 @@ __global__ __launch_bounds__(256) void solve_kernel(SolveArgs A) {
 #pragma unroll
         for (int r = 0; r < 4; r++) w[r] = A.u0[t * kTilePix + (4 * grp + r) * kTile + col];
         apply_k<GEN>(w, T, wg);
-        float red2[2];
-        red2[0] = ll_lane(w, gm, c, lin, wll);
-        red2[1] = w[0] * wg[0] + w[1] * wg[1] + (w[2] * wg[2] + w[3] * wg[3]);
-        float both[4] = {red[0], red[1], red2[0], red2[1]};
+        float both[4];
+        both[0] = red[0];
+        both[1] = red[1];
+        both[2] = ll_lane(w, gm, c, lin, wll);
+        both[3] = w[0] * wg[0] + w[1] * wg[1] + (w[2] * wg[2] + w[3] * wg[3]);
         wave_sum<4>(both);
         const float f_plain = both[0] + 0.5f * alpha * both[1];
         const float f_warm = both[2] + 0.5f * alpha * both[3];
@@ -449,9 +492,8 @@ __global__ __launch_bounds__(256) void solve_kernel(SolveArgs A) {
           f = f_plain;
         }
       } else {
-        float r2[2] = {red[0], red[1]};
-        wave_sum<2>(r2);
-        f = r2[0] + 0.5f * alpha * r2[1];
+        wave_sum<2>(red);
+        f = red[0] + 0.5f * alpha * red[1];
       }
     }
 #pragma unroll
@@ -470,8 +512,8 @@ __global__ __launch_bounds__(256) void solve_kernel(SolveArgs A) {
     bool done = sqrtf(gg) <= A.eps * fmaxf(1.0f, sqrtf(uu));
     if (done) status = NCS_SUCCESS;
 
-    float S[M][4], Y[M][4], rho[M], al[M];
-    int nh = 0;
+    float rho[M], al[M];  // rho[j], al[j]: j-th newest pair
+    int nh = 0, head = M - 1;
     float dvec[4];
 #pragma unroll
     for (int r = 0; r < 4; r++) dvec[r] = -g[r];
@@ -499,51 +541,53 @@ __global__ __launch_bounds__(256) void solve_kernel(SolveArgs A) {
         break;
       }
 
-      float v0[4];
-      bool grd0[4];
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        float w0 = u[r] + gm[r];
-        grd0[r] = !(w0 >= kFitMin);
-        v0[r] = fmaxf(w0, kFitMin);
-      }
-
-      // line search
-      float ut[4], gt[4], phi = 0.0f;
-      int ls = LS_CONTINUE;
-      MtState S_;
-      if (A.ls == NCS_LS_MORE_THUENTE) mt_init(S_, dg0, A);
-      float tstep = step;
-      int trials = 0;
-      for (;;) {
-        if (A.ls == NCS_LS_MORE_THUENTE) tstep = mt_pre(S_, tstep, A);
-        float red[2] = {0.0f, 0.0f};
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-          const float td = tstep * dvec[r];
-          ut[r] = u[r] + td;
-          const float w = ut[r] + gm[r];
-          const float vt = fmaxf(w, kFitMin);
-          const float delta = (grd0[r] || !(w >= kFitMin)) ? vt - v0[r] : td;
-          const float lg = log1p_acc(delta * rcpf(v0[r]));
-          red[0] += lin[r] * td - c[r] * lg;
-          gt[r] = lin[r] - c[r] * rcpf(vt);
-          red[1] += gt[r] * dvec[r];
-        }
-        wave_sum<2>(red);
+      // line search; the first trial is taken at the clamped initial step
+      float gt[4], phi;
+      float tstep = fminf(fmaxf(step, A.min_step), A.max_step);
+      int ls;
+      {
+        float red[2];
+        ll_trial<PAD>(tstep, u, dvec, gm, c, lin, gt, red);
         evals++;
-        trials++;
         phi = red[0] + alpha * tstep * (a1 + 0.5f * tstep * a2);
         const float dphi = red[1] + alpha * (a1 + tstep * a2);
         if (A.ls == NCS_LS_MORE_THUENTE) {
-          ls = mt_post(S_, tstep, phi, dphi, A);
-          if (ls != LS_CONTINUE) break;
+          if (A.max_ls > 1 && tstep != A.max_step && tstep != A.min_step &&
+              phi <= A.ftol * dg0 * tstep && fabsf(dphi) <= A.gtol * (-dg0)) {
+            ls = LS_ACCEPT;  // what the full search decides on this trial
+          } else {
+            // Full More-Thuente search (about once per tile).  Its state is
+            // kept in LDS and the memory clobbers stop the compiler from
+            // holding it in VGPRs across the vector trials.
+            MtState &S_ = mt_lds;
+            mt_init(S_, dg0, A);
+            S_.stmin = 0.0f;
+            S_.stmax = 5.0f * tstep;
+            ls = mt_post(S_, tstep, phi, dphi, A);
+            while (ls == LS_CONTINUE) {
+              tstep = mt_pre(S_, tstep, A);
+              asm volatile("" ::: "memory");
+              ll_trial<PAD>(tstep, u, dvec, gm, c, lin, gt, red);
+              asm volatile("" ::: "memory");
+              evals++;
+              phi = red[0] + alpha * tstep * (a1 + 0.5f * tstep * a2);
+              const float dp = red[1] + alpha * (a1 + tstep * a2);
+              ls = mt_post(S_, tstep, phi, dp, A);
+            }
+          }
         } else {
           // Armijo backtracking (ncs.cl:979-1035)
-          if (phi <= A.ftol * dg0 * tstep) { ls = LS_ACCEPT; break; }
-          tstep *= A.stepm;
-          if (!(tstep >= A.min_step)) { ls = LS_FAIL_MINSTEP; break; }
-          if (trials >= A.max_ls) { ls = LS_FAIL_PRECISION; break; }
+          int trials = 1;
+          for (;;) {
+            if (phi <= A.ftol * dg0 * tstep) { ls = LS_ACCEPT; break; }
+            tstep *= A.stepm;
+            if (!(tstep >= A.min_step)) { ls = LS_FAIL_MINSTEP; break; }
+            if (trials >= A.max_ls) { ls = LS_FAIL_PRECISION; break; }
+            ll_trial<PAD>(tstep, u, dvec, gm, c, lin, gt, red);
+            evals++;
+            trials++;
+            phi = red[0] + alpha * tstep * (a1 + 0.5f * tstep * a2);
+          }
         }
       }
       if (ls != LS_ACCEPT) {
@@ -554,9 +598,10 @@ __global__ __launch_bounds__(256) void solve_kernel(SolveArgs A) {
       }
 
       // accept u + t d
-      float s[4], gnew[4], gncn[4], y[4];
+      float ut[4], s[4], gnew[4], gncn[4], y[4];
 #pragma unroll
       for (int r = 0; r < 4; r++) {
+        ut[r] = u[r] + tstep * dvec[r];
         s[r] = ut[r] - u[r];
         gncn[r] = fmaf(tstep, kd[r], gnc[r]);
       }
@@ -600,42 +645,51 @@ __global__ __launch_bounds__(256) void solve_kernel(SolveArgs A) {
       if (!moved) { status = NCS_REACHED_MAXPRECISION; break; }
       if (k == A.max_it) { status = NCS_REACHED_MAXITERS; break; }
 
-      // history update (newest pair at index M-1)
+      // history update: newest pair into the LDS rings
       if (ys > 0.0f) {
+        head = (head + 1 == M) ? 0 : head + 1;
+        Sb[head * 64 + lane] = make_float4(s[0], s[1], s[2], s[3]);
+        Yb[head * 64 + lane] = make_float4(y[0], y[1], y[2], y[3]);
 #pragma unroll
-        for (int j = 0; j < M - 1; j++) {
-#pragma unroll
-          for (int r = 0; r < 4; r++) { S[j][r] = S[j + 1][r]; Y[j][r] = Y[j + 1][r]; }
-          rho[j] = rho[j + 1];
-        }
-#pragma unroll
-        for (int r = 0; r < 4; r++) { S[M - 1][r] = s[r]; Y[M - 1][r] = y[r]; }
-        rho[M - 1] = 1.0f / ys;
+        for (int j = M - 1; j > 0; j--) rho[j] = rho[j - 1];
+        rho[0] = 1.0f / ys;
         ys_last = ys;
         yy_last = yy;
         nh = min(nh + 1, M);
       }
 
-      // two-loop recursion
+      // two-loop recursion, newest to oldest then back
 #pragma unroll
       for (int r = 0; r < 4; r++) dvec[r] = -g[r];
 #pragma unroll
-      for (int j = M - 1; j >= 0; j--) {
-        if (j >= M - nh) {
-          al[j] = rho[j] * wave_dot(S[j], dvec);
-#pragma unroll
-          for (int r = 0; r < 4; r++) dvec[r] = fmaf(-al[j], Y[j][r], dvec[r]);
+      for (int j = 0; j < M; j++) {
+        if (j < nh) {
+          const int slot = (head - j < 0) ? head - j + M : head - j;
+          const float4 sv = Sb[slot * 64 + lane];
+          const float4 yv = Yb[slot * 64 + lane];
+          const float sj[4] = {sv.x, sv.y, sv.z, sv.w};
+          al[j] = rho[j] * wave_dot(sj, dvec);
+          dvec[0] = fmaf(-al[j], yv.x, dvec[0]);
+          dvec[1] = fmaf(-al[j], yv.y, dvec[1]);
+          dvec[2] = fmaf(-al[j], yv.z, dvec[2]);
+          dvec[3] = fmaf(-al[j], yv.w, dvec[3]);
         }
       }
       const float hscale = (nh > 0) ? ys_last / yy_last : rsqrtf(gg);
 #pragma unroll
       for (int r = 0; r < 4; r++) dvec[r] *= hscale;
 #pragma unroll
-      for (int j = 0; j < M; j++) {
-        if (j >= M - nh) {
-          const float b = rho[j] * wave_dot(Y[j], dvec);
-#pragma unroll
-          for (int r = 0; r < 4; r++) dvec[r] = fmaf(al[j] - b, S[j][r], dvec[r]);
+      for (int j = M - 1; j >= 0; j--) {
+        if (j < nh) {
+          const int slot = (head - j < 0) ? head - j + M : head - j;
+          const float4 sv = Sb[slot * 64 + lane];
+          const float4 yv = Yb[slot * 64 + lane];
+          const float yj[4] = {yv.x, yv.y, yv.z, yv.w};
+          const float b = al[j] - rho[j] * wave_dot(yj, dvec);
+          dvec[0] = fmaf(b, sv.x, dvec[0]);
+          dvec[1] = fmaf(b, sv.y, dvec[1]);
+          dvec[2] = fmaf(b, sv.z, dvec[2]);
+          dvec[3] = fmaf(b, sv.w, dvec[3]);
         }
       }
       step = 1.0f;
@@ -726,46 +780,50 @@ __global__ void reset_queue_kernel(unsigned int *q) {
   if (threadIdx.x == 0) *q = 0u;
 }
 
-// Instantiations: history 1..8, with and without the odd-mask coupling.
-#define NCS_INST(M)                                               \
-  template __global__ void solve_kernel<M, false>(SolveArgs);     \
-  template __global__ void solve_kernel<M, true>(SolveArgs);
-NCS_INST(1) NCS_INST(2) NCS_INST(3) NCS_INST(4)
-NCS_INST(5) NCS_INST(6) NCS_INST(7) NCS_INST(8)
-#undef NCS_INST
-
+// Instantiations: history 1..8, with and without the odd-mask coupling and
+// zero-padded pixels.
 typedef void (*SolveKernelFn)(SolveArgs);
 
-static SolveKernelFn solve_kernel_for(int m, bool general) {
+template <int M>
+static SolveKernelFn pick(bool general, bool pad) {
+  if (general) return pad ? solve_kernel<M, true, true> : solve_kernel<M, true, false>;
+  return pad ? solve_kernel<M, false, true> : solve_kernel<M, false, false>;
+}
+
+static SolveKernelFn solve_kernel_for(int m, bool general, bool pad) {
   switch (m) {
-#define NCS_CASE(M) \
-  case M: return general ? solve_kernel<M, true> : solve_kernel<M, false>;
-    NCS_CASE(1) NCS_CASE(2) NCS_CASE(3) NCS_CASE(4)
-    NCS_CASE(5) NCS_CASE(6) NCS_CASE(7) NCS_CASE(8)
-#undef NCS_CASE
+    case 1: return pick<1>(general, pad);
+    case 2: return pick<2>(general, pad);
+    case 3: return pick<3>(general, pad);
+    case 4: return pick<4>(general, pad);
+    case 5: return pick<5>(general, pad);
+    case 6: return pick<6>(general, pad);
+    case 7: return pick<7>(general, pad);
+    case 8: return pick<8>(general, pad);
     default: return nullptr;
   }
 }
 
-// Workgroups of 4 waves one CU holds at once for this instantiation (0 if
-// the history length is unsupported).
-int solve_blocks_per_cu(int m, bool general) {
-  SolveKernelFn fn = solve_kernel_for(m, general);
+static size_t solve_lds_bytes(int m) { return (size_t)m * 2 * 64 * sizeof(float4); }
+
+// Waves of this instantiation one CU holds at once (0 if unsupported).
+int solve_waves_per_cu(int m, bool general, bool pad) {
+  SolveKernelFn fn = solve_kernel_for(m, general, pad);
   if (!fn) return 0;
   int blocks = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-          &blocks, reinterpret_cast<const void *>(fn), kWave * kWavesPerBlock, 0) !=
+          &blocks, reinterpret_cast<const void *>(fn), kWave, solve_lds_bytes(m)) !=
       hipSuccess)
     return 0;
   return blocks;
 }
 
-hipError_t launch_solve(int m, bool general, unsigned int blocks, hipStream_t s,
+hipError_t launch_solve(int m, bool general, bool pad, unsigned int waves, hipStream_t s,
                         const SolveArgs &a) {
-  SolveKernelFn fn = solve_kernel_for(m, general);
+  SolveKernelFn fn = solve_kernel_for(m, general, pad);
   if (!fn) return hipErrorInvalidValue;
   hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(64), 0, s, a.queue);
-  hipLaunchKernelGGL(fn, dim3(blocks), dim3(kWave * kWavesPerBlock), 0, s, a);
+  hipLaunchKernelGGL(fn, dim3(waves), dim3(kWave), solve_lds_bytes(m), s, a);
   return hipGetLastError();
 }
 

@@ -99,6 +99,11 @@ def load():
                                          ctypes.POINTER(NCSOParam), ctypes.c_int,
                                          vp, vp, vp, vp, vp]
     lib.ncso_solve_tiles_f32.restype = ctypes.c_int
+    lib.ncso_solve_tiles_masked_f32.argtypes = [f32, f32, f32, f32, ctypes.c_double,
+                                                ctypes.c_int, ctypes.c_int,
+                                                ctypes.POINTER(NCSOParam), ctypes.c_int,
+                                                vp, vp, vp, vp]
+    lib.ncso_solve_tiles_masked_f32.restype = ctypes.c_int
     lib.ncso_eval_cost_f32.argtypes = [f32, f32, f32, f32, ctypes.c_double,
                                        ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                        vp, vp, vp]
@@ -166,6 +171,26 @@ def solve_tiles(data, gamma, mask_fft, alpha, u0=None, param=None,
     if rc != 0:
         raise ValueError("oracle solve_tiles rejected its arguments")
     return dict(u=u, cost=cost, status=status, iters=iters, evals=evals)
+
+
+def solve_tiles_masked(data, gamma, lin, mask_fft, alpha, param=None, nthreads=0):
+    """float32 tiles with a 0/1 likelihood weight per pixel (block tiling)."""
+    lib = load()
+    data = numpy.ascontiguousarray(data, numpy.float32)
+    nt, r = data.shape[0], data.shape[1]
+    u = numpy.zeros_like(data)
+    cost = numpy.zeros(nt, numpy.float64)
+    status = numpy.zeros(nt, numpy.int32)
+    iters = numpy.zeros(nt, numpy.int32)
+    prm = param if param is not None else default_param()
+    rc = lib.ncso_solve_tiles_masked_f32(
+        data, numpy.ascontiguousarray(gamma, numpy.float32),
+        numpy.ascontiguousarray(lin, numpy.float32),
+        numpy.ascontiguousarray(mask_fft, numpy.float32), float(alpha), r, nt,
+        ctypes.byref(prm), int(nthreads), _ptr(u), _ptr(cost), _ptr(status), _ptr(iters))
+    if rc != 0:
+        raise ValueError("oracle solve_tiles_masked rejected its arguments")
+    return dict(u=u, cost=cost, status=status, iters=iters)
 
 
 def eval_cost_f32(u, data, gamma, mask_fft, alpha, nthreads=0):

@@ -30,6 +30,7 @@ struct ncsSubRegion {
   double *g;                   /* c2r output (ncs.c:433) */
   double *cos_tab, *sin_tab;   /* cos/sin(2*pi*m/r), m = 0..r-1 */
   double *w_re, *w_im;         /* r x r DFT workspace */
+  const double *lin;           /* per-pixel likelihood weight (NULL = all 1) */
   ncso_param param;
   int last_iters, last_evals;
   double last_cost;
@@ -140,6 +141,7 @@ void ncsSRCalcLLGradient(ncsSubRegion *sr, double *gradient) {
     double t1 = sr->data[i] + sr->gamma[i];
     double t2 = sr->u[i] + sr->gamma[i];
     if (t2 < FITMIN) t2 = FITMIN;
+    if (sr->lin && sr->lin[i] == 0.0) { gradient[i] = 0.0; continue; }
     gradient[i] = 1.0 - t1 / t2;
   }
 }
@@ -152,6 +154,7 @@ double ncsSRCalcLogLikelihood(ncsSubRegion *sr) {
     double t1 = sr->data[i] + sr->gamma[i];
     double t2 = sr->u[i] + sr->gamma[i];
     if (t2 < FITMIN) t2 = FITMIN;
+    if (sr->lin && sr->lin[i] == 0.0) continue;
     sum += sr->u[i] - t1 * log(t2);
   }
   return sum;
@@ -735,6 +738,52 @@ void ncso_eval_cost_f32(const float *u, const float *data, const float *gamma,
     free(g);
     ncsSRCleanup(sr);
   }
+}
+
+/* Tiles whose pixels may be excluded from the likelihood (lin = 0): the
+ * zero-padded edge blocks of the non-overlapping block tiling.  lin is
+ * [ntiles][r*r] with 0/1 entries; padded pixels stay variables of the NC
+ * term only.  Not a reference feature: the reference always replicates
+ * edges (ncs.c:87-113). */
+int ncso_solve_tiles_masked_f32(const float *data, const float *gamma,
+                                const float *lin, const float *mask_fft,
+                                double alpha, int r, int ntiles,
+                                const ncso_param *p, int nthreads, float *u_out,
+                                double *cost_out, int *status_out,
+                                int *iters_out) {
+  if (r <= 0 || (r % 2) != 0 || ntiles < 0 || r * r > 1024) return -1;
+  int n = r * r;
+  ncso_param prm;
+  if (p) prm = *p; else ncso_param_init(&prm);
+  double mask[1024];
+  for (int i = 0; i < n; i++) mask[i] = mask_fft[i];
+  int nt = resolve_threads(nthreads);
+#pragma omp parallel num_threads(nt)
+  {
+    ncsSubRegion *sr = ncsSRInitialize(r);
+    ncsSRSetOTFMask(sr, mask);
+    sr->param = prm;
+    double *w = (double *)malloc(sizeof(double) * n);
+    sr->lin = w;
+#pragma omp for schedule(dynamic, 4)
+    for (int t = 0; t < ntiles; t++) {
+      for (int i = 0; i < n; i++) {
+        sr->data[i] = data[(size_t)t * n + i];
+        sr->gamma[i] = gamma[(size_t)t * n + i];
+        w[i] = lin[(size_t)t * n + i];
+      }
+      int st = ncsSRSolve(sr, alpha, 0);
+      if (u_out)
+        for (int i = 0; i < n; i++) u_out[(size_t)t * n + i] = (float)sr->u[i];
+      if (cost_out) cost_out[t] = sr->last_cost;
+      if (status_out) status_out[t] = st;
+      if (iters_out) iters_out[t] = sr->last_iters;
+    }
+    sr->lin = NULL;
+    free(w);
+    ncsSRCleanup(sr);
+  }
+  return 0;
 }
 
 /* ------------------------------------------------------------------ */

@@ -139,6 +139,15 @@ int ncso_solve_tiles_f32(const float *data, const float *gamma,
                          float *u_out, double *cost_out, int *status_out,
                          int *iters_out, int *evals_out);
 
+/* FP32 tiles with a per-pixel likelihood weight lin in {0, 1} (zero-padded
+ * pixels of the block tiling); lin = 1 everywhere is ncsSRSolve. */
+int ncso_solve_tiles_masked_f32(const float *data, const float *gamma,
+                                const float *lin, const float *mask_fft,
+                                double alpha, int r, int ntiles,
+                                const ncso_param *p, int nthreads, float *u_out,
+                                double *cost_out, int *status_out,
+                                int *iters_out);
+
 /* Batch cost evaluation of FP32 tiles in FP64 (error reporting). */
 void ncso_eval_cost_f32(const float *u, const float *data, const float *gamma,
                         const float *mask_fft, double alpha, int r, int ntiles,

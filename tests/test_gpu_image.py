@@ -140,3 +140,43 @@ def test_reference_warnings_and_errors():
     with pytest.warns(UserWarning):
         ncs.reduceNoise([fr["data"]], fr["gamma"], masks.otf_mask(), 0.1,
                         params=_lib.default_params(max_iterations=1))
+
+
+def test_block_geometry_ragged_edges_match_masked_oracle():
+    """Non-overlapping 16 x 16 blocks with zero-padded edge blocks whose padded
+    pixels are excluded from the likelihood: per-tile final cost against the
+    oracle's masked-likelihood solve of the same padded tiles."""
+    _say("block geometry, ragged edges")
+    fr = sim.make_frame(17, (45, 70))
+    mask = masks.otf_mask()
+    mf = masks.to_fft_order(mask).astype(numpy.float32)
+    out, info = ncs.reduceNoise([fr["data"]], fr["gamma"], mask, 0.15, geometry="blocks",
+                                return_info=True)
+    H, W = fr["data"].shape
+    tiles_d, tiles_g, tiles_l = [], [], []
+    for by in range(0, H, 16):
+        for bx in range(0, W, 16):
+            d = numpy.zeros((16, 16), numpy.float32)
+            g = numpy.zeros((16, 16), numpy.float32)
+            l = numpy.zeros((16, 16), numpy.float32)
+            hh, ww = min(16, H - by), min(16, W - bx)
+            d[:hh, :ww] = fr["data"][by:by + hh, bx:bx + ww]
+            g[:hh, :ww] = fr["gamma"][by:by + hh, bx:bx + ww]
+            l[:hh, :ww] = 1.0
+            tiles_d.append(d)
+            tiles_g.append(g)
+            tiles_l.append(l)
+    ref = oracle.solve_tiles_masked(numpy.stack(tiles_d), numpy.stack(tiles_g),
+                                    numpy.stack(tiles_l), mf, 0.15)
+    assert len(info["cost"]) == len(ref["cost"]) == 3 * 5
+    rel = numpy.abs(info["cost"] - ref["cost"]) / numpy.abs(ref["cost"])
+    _say("blocks: tile cost max rel", rel.max(), "status", numpy.unique(info["status"]))
+    assert numpy.all(info["status"] == 0)
+    assert rel.max() <= 1e-5
+    # stitched pixels are the valid part of each solved block
+    k = 0
+    for by in range(0, H, 16):
+        for bx in range(0, W, 16):
+            hh, ww = min(16, H - by), min(16, W - bx)
+            assert numpy.max(numpy.abs(out[0][by:by + hh, bx:bx + ww] - ref["u"][k][:hh, :ww])) < 0.05
+            k += 1
