@@ -172,6 +172,12 @@ void ncsReduceNoise(double *ncs_image, double *image, double *gamma,
 /* Sets the context's network (host FP32 parameters) for NCS_START_LEARNED. */
 int ncs_hip_net_set_weights(ncs_ctx *ctx, const float *params_f32);
 
+/* The starts u0 = d + sigma * net(x) of every window of device frames, as
+ * NCS_START_LEARNED computes them, into u0_tiles (device [tiles][256]). */
+int ncs_hip_net_predict_frames_device(ncs_ctx *ctx, const ncs_frame *frames,
+                                      int n_frames, float *u0_tiles,
+                                      void *stream);
+
 /* Forward on device tensors: x [n][2][256] normalised inputs (FP32) ->
  * out [n][256] (FP32).  acts (device, may be NULL) receives the post-ReLU
  * activations of layers 1-3 as BF16 [3][n][256][32] for the backward pass.

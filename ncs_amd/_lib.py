@@ -90,6 +90,7 @@ SIGNATURES = {
                                         _vp]),
     "ncs_hip_eval_tiles": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _i, _f]),
     "ncs_hip_net_set_weights": (_i, [_vp, _vp]),
+    "ncs_hip_net_predict_frames_device": (_i, [_vp, _F, _i, _vp, _vp]),
     "ncs_hip_net_forward": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _vp]),
     "ncs_hip_net_backward": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp]),
     "ncsReduceNoise": (None, [_vp, _vp, _vp, _vp, ctypes.c_double, _i, _i, _i]),
@@ -262,25 +263,38 @@ class Context(object):
             cost_ptr, int(n), float(alpha), ctypes.byref(params), stream_ptr),
             "ncs_hip_solve_tiles_device")
 
+    def predict_frames_device(self, frames, u0_ptr, stream_ptr=None):
+        """Learned starts of every window of device frames into u0 (device)."""
+        check(self.lib.ncs_hip_net_predict_frames_device(self.handle, frame_array(frames),
+                                                         len(frames), u0_ptr, stream_ptr),
+              "ncs_hip_net_predict_frames_device")
+
     def reduce_frames_device(self, frames, params, u0_ptr=None, iters_ptr=None,
                              status_ptr=None, cost_ptr=None, stream_ptr=None):
         """frames: list of dicts(data, gamma, out, height, width, alpha, geometry,
         [data_stride, gamma_stride, out_stride]) with device pointers; async."""
         n = len(frames)
-        arr = (Frame * n)()
-        for i, f in enumerate(frames):
-            a = arr[i]
-            a.data, a.gamma, a.out = f["data"], f["gamma"], f.get("out")
-            a.height, a.width = f["height"], f["width"]
-            a.data_stride = f.get("data_stride", 0)
-            a.gamma_stride = f.get("gamma_stride", 0)
-            a.out_stride = f.get("out_stride", 0)
-            a.alpha = float(f["alpha"])
-            a.geometry = int(f.get("geometry", GEOM_REFERENCE))
+        arr = frame_array(frames)
         check(self.lib.ncs_hip_reduce_frames_device(self.handle, arr, n, ctypes.byref(params),
                                                     u0_ptr, iters_ptr, status_ptr, cost_ptr,
                                                     stream_ptr),
               "ncs_hip_reduce_frames_device")
+
+
+def frame_array(frames):
+    """ctypes array of ncs_frame from dicts of device pointers and sizes."""
+    n = len(frames)
+    arr = (Frame * n)()
+    for i, f in enumerate(frames):
+        a = arr[i]
+        a.data, a.gamma, a.out = f["data"], f["gamma"], f.get("out")
+        a.height, a.width = f["height"], f["width"]
+        a.data_stride = f.get("data_stride", 0)
+        a.gamma_stride = f.get("gamma_stride", 0)
+        a.out_stride = f.get("out_stride", 0)
+        a.alpha = float(f["alpha"])
+        a.geometry = int(f.get("geometry", GEOM_REFERENCE))
+    return arr
 
 
 _contexts = {}

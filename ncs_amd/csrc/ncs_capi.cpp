@@ -628,6 +628,24 @@ static int packs_for(ncs_ctx *ctx, const float *params, hipStream_t s, const voi
   return NCS_OK;
 }
 
+extern "C" int ncs_hip_net_predict_frames_device(ncs_ctx *ctx, const ncs_frame *frames,
+                                                 int n_frames, float *u0_tiles, void *stream) {
+  if (!ctx || !frames || n_frames <= 0 || !u0_tiles) return fail(NCS_E_ARG, "bad arguments");
+  if (!ctx->has_net) return fail(NCS_E_NOWEIGHTS, "no network weights set");
+  std::vector<FrameDev> fd;
+  long long total = 0;
+  int rc = build_frames(frames, n_frames, fd, total);
+  if (rc) return rc;
+  HIPCHK(set_device(ctx));
+  hipStream_t s = (hipStream_t)stream;
+  rc = upload_frames(ctx, fd, s);
+  if (rc) return rc;
+  int grid = (int)std::max<long long>(1, std::min<long long>(total, (long long)ctx->cus * 4));
+  HIPCHK(net_warm(ctx->frames.as<FrameDev>(), n_frames, total, u0_tiles, ctx->net_packs.p,
+                  ctx->net_params.as<float>(), grid, s));
+  return NCS_OK;
+}
+
 extern "C" int ncs_hip_net_forward(ncs_ctx *ctx, const float *params, const float *x, float *out,
                                    void *acts, int n, void *stream) {
   if (!ctx || !x || !out || n < 0) return fail(NCS_E_ARG, "bad arguments");

@@ -154,11 +154,9 @@ def _frame_windows(args):
     return reference_windows(fr["data"]), reference_windows(fr["gamma"])
 
 
-def make_pairs(n_tiles, seed0=10_000_000, frame_shape=(512, 512), alpha=0.1, mask=None,
-               params=None, device=0, workers=8, log=print):
-    """(d, gamma, u*) tiles from synthetic frames with seeds >= seed0, u*
-    solved by the plain GPU solver (history 5, eps 1e-5)."""
-    import torch
+def gen_windows(n_tiles, seed0=10_000_000, frame_shape=(512, 512), workers=8, log=print):
+    """Host windows (d, gamma) [n, 16, 16] of synthetic frames with seeds
+    >= seed0.  Uses a process pool: call it before the GPU is initialised."""
     per = _lib.tile_count(*frame_shape)
     nfr = (n_tiles + per - 1) // per
     t0 = time.time()
@@ -171,7 +169,13 @@ def make_pairs(n_tiles, seed0=10_000_000, frame_shape=(512, 512), alpha=0.1, mas
         res = [_frame_windows(j) for j in jobs]
     d = numpy.concatenate([r[0] for r in res])[:n_tiles].astype(numpy.float32)
     g = numpy.concatenate([r[1] for r in res])[:n_tiles].astype(numpy.float32)
-    log("pairs: %d tiles from %d frames in %.1f s" % (len(d), nfr, time.time() - t0))
+    log("windows: %d tiles from %d frames in %.1f s" % (len(d), nfr, time.time() - t0))
+    return d, g
+
+
+def solve_pairs(d, g, alpha=0.1, mask=None, params=None, device=0, log=print):
+    """Device tensors (d, gamma, u*) with u* from the plain GPU solver."""
+    import torch
     ctx = _lib.context(device)
     m = mask if mask is not None else masks.otf_mask()
     ctx.set_otf_mask(masks.to_fft_order(m).astype(numpy.float32))
@@ -186,6 +190,13 @@ def make_pairs(n_tiles, seed0=10_000_000, frame_shape=(512, 512), alpha=0.1, mas
     torch.cuda.synchronize()
     log("pairs solved: iterations mean %.2f" % it.float().mean().item())
     return td, tg, u
+
+
+def make_pairs(n_tiles, seed0=10_000_000, frame_shape=(512, 512), alpha=0.1, mask=None,
+               params=None, device=0, workers=1, log=print):
+    """(d, gamma, u*) device tensors: gen_windows + solve_pairs."""
+    d, g = gen_windows(n_tiles, seed0, frame_shape, workers, log)
+    return solve_pairs(d, g, alpha, mask, params, device, log)
 
 
 def train(d, g, u, steps=3000, batch=1024, lr=2e-3, seed=0, log=print, params0=None):
@@ -235,3 +246,63 @@ def set_context_weights(params, device=0):
     _lib.check(ctx.lib.ncs_hip_net_set_weights(ctx.handle, ctypes.c_void_p(p.ctypes.data)),
                "ncs_hip_net_set_weights")
     return ctx
+
+
+def main(argv=None):
+    """python -m ncs_amd.warmstart train [--tiles N] [--steps S] [--out PATH]
+
+    Generates (tile, NCS solution) pairs with the plain GPU solver from
+    synthetic frames (seeds >= 1e7, disjoint from the benchmark's seeds
+    1..10), trains the net with its HIP kernels and saves the FP32 weights."""
+    import argparse
+    import json
+
+    import torch
+    ap = argparse.ArgumentParser()
+    ap.add_argument("cmd", choices=["train"])
+    ap.add_argument("--tiles", type=int, default=200_000)
+    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--batch", type=int, default=1024)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--alpha", type=float, default=0.1)
+    ap.add_argument("--out", default=DEFAULT_WEIGHTS)
+    ap.add_argument("--workers", type=int, default=min(16, os.cpu_count() or 1))
+    a = ap.parse_args(argv)
+    log = lambda *x: print(*x, flush=True)
+    # every process pool runs before the GPU is touched
+    dn, gn = gen_windows(a.tiles, workers=a.workers, log=log)
+    dh, gh = gen_windows(8192, seed0=1, workers=a.workers, log=log)
+    d, g, u = solve_pairs(dn, gn, alpha=a.alpha, log=log)
+    params, hist = train(d, g, u, steps=a.steps, batch=a.batch, lr=a.lr, log=log)
+    save(params.cpu().numpy(), a.out)
+    # held-out tiles (seeds 1..): iterations of the plain and the learned start
+    dt, gt_, ut = solve_pairs(dh, gh, alpha=a.alpha, log=log)
+    u0 = predict_u0(params, dt, gt_).detach().contiguous()
+    ctx = _lib.context(0)
+    n = dt.shape[0]
+    res = {}
+    for name, start in (("plain", None), ("learned", u0)):
+        out = torch.zeros_like(dt)
+        it = torch.zeros(n, dtype=torch.int32, device=dt.device)
+        st = torch.zeros_like(it)
+        cs = torch.zeros(n, device=dt.device)
+        ctx.solve_tiles_device(dt.data_ptr(), gt_.data_ptr(),
+                               None if start is None else start.data_ptr(), out.data_ptr(),
+                               it.data_ptr(), st.data_ptr(), cs.data_ptr(), n, a.alpha,
+                               _lib.default_params(), _stream())
+        torch.cuda.synchronize()
+        res[name] = dict(iterations=float(it.float().mean().item()), cost=cs.cpu().numpy())
+    rel = numpy.abs(res["learned"]["cost"] - res["plain"]["cost"]) / numpy.abs(res["plain"]["cost"])
+    err_net = float(((u0 - ut).abs().mean() / ut.abs().mean()).item())
+    err_plain = float(((dt - ut).abs().mean() / ut.abs().mean()).item())
+    summary = dict(weights=a.out, tiles=a.tiles, steps=a.steps, loss=hist,
+                   held_out_iterations_plain=res["plain"]["iterations"],
+                   held_out_iterations_learned=res["learned"]["iterations"],
+                   held_out_cost_max_rel=float(rel.max()),
+                   held_out_rel_error_start_net=err_net, held_out_rel_error_start_plain=err_plain)
+    log(json.dumps(summary))
+    return summary
+
+
+if __name__ == "__main__":
+    main()

@@ -124,11 +124,10 @@ def test_training_lowers_the_loss_and_saves_iterations():
     start needs fewer iterations than the plain start at matched cost."""
     import torch
     _say("short training run")
-    d, g, u = warmstart.make_pairs(4096, seed0=20_000_000, frame_shape=(128, 128), workers=1,
-                                   log=_say)
+    d, g, u = warmstart.make_pairs(4096, seed0=20_000_000, frame_shape=(128, 128), log=_say)
     params, hist = warmstart.train(d, g, u, steps=300, batch=512, lr=2e-3, log=_say)
     assert hist[-1][1] < 0.7 * hist[0][1]
-    dt, gt_, ut = warmstart.make_pairs(1024, seed0=1, frame_shape=(128, 128), workers=1, log=_say)
+    dt, gt_, ut = warmstart.make_pairs(1024, seed0=1, frame_shape=(128, 128), log=_say)
     u0 = warmstart.predict_u0(params, dt, gt_).detach()
     ctx = _lib.context(0)
     n = dt.shape[0]
