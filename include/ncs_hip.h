@@ -70,6 +70,10 @@ typedef struct {
   int start;           /* NCS_START_* */
   int fallback;        /* warm starts: keep u0 = data where f(data) <= f(u0) */
   int waves_per_cu;    /* persistent grid size; 0 = autotuned */
+  int precond;         /* 1: initial inverse Hessian H0 = R^T diag(1/(kappa +
+                          alpha*2q)) R in the real DFT basis R, kappa = tile
+                          mean of (d+gamma)/(u+gamma)^2 (default); 0: the
+                          scalar s.y/y.y of liblbfgs and ncs.cl:1123 */
 } ncs_params;
 
 typedef struct {
@@ -88,7 +92,7 @@ typedef struct ncs_ctx ncs_ctx;
 
 /* Fill defaults: max_iterations 200, history 5, epsilon 1e-5, More-Thuente
  * (ftol 1e-4, gtol 0.9, xtol 1e-7, max_linesearch 20, steps 1e-20..1e20),
- * Armijo stepm 0.5, plain start, fallback on, autotuned grid. */
+ * Armijo stepm 0.5, plain start, fallback on, autotuned grid, spectral H0. */
 void ncs_hip_default_params(ncs_params *p);
 
 int ncs_hip_create(int device, ncs_ctx **out);

@@ -50,7 +50,8 @@ class Params(ctypes.Structure):
                 ("stepm", ctypes.c_float),
                 ("start", ctypes.c_int),
                 ("fallback", ctypes.c_int),
-                ("waves_per_cu", ctypes.c_int)]
+                ("waves_per_cu", ctypes.c_int),
+                ("precond", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -138,12 +139,22 @@ def default_params(**overrides):
     return p
 
 
+def liblbfgs_params(**overrides):
+    """The reference C library's solver semantics (liblbfgs 1.10 defaults:
+    history 6, eps 1e-5, More-Thuente, scalar initial Hessian)."""
+    p = default_params(history=6, precond=0)
+    for k, v in overrides.items():
+        setattr(p, k, v)
+    return p
+
+
 def kernel_semantics_params(**overrides):
     """Parameters of the reference GPU kernel (opencl/ncs.cl:58-63): Armijo
     backtracking with c1 = 1e-4, step x0.5 down to 1e-6, history 8,
     epsilon 1e-4, at most 200 iterations."""
     p = default_params(linesearch=LS_ARMIJO, ftol=1e-4, stepm=0.5, min_step=1e-6,
-                       history=8, epsilon=1e-4, max_iterations=200, max_linesearch=64)
+                       history=8, epsilon=1e-4, max_iterations=200, max_linesearch=64,
+                       precond=0)
     for k, v in overrides.items():
         setattr(p, k, v)
     return p

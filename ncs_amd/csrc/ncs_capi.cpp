@@ -131,6 +131,7 @@ extern "C" void ncs_hip_default_params(ncs_params *p) {
   p->start = NCS_START_PLAIN;
   p->fallback = 1;
   p->waves_per_cu = 0;
+  p->precond = 1;
 }
 
 extern "C" const char *ncs_hip_last_error(void) { return g_last_error.c_str(); }
@@ -289,6 +290,7 @@ static int check_params(const ncs_params *p) {
     return fail(NCS_E_ARG, "stepm must be in (0, 1)");
   if (!(p->xtol >= 0.0f)) return fail(NCS_E_ARG, "xtol must be >= 0");
   if (p->waves_per_cu < 0 || p->waves_per_cu > 64) return fail(NCS_E_ARG, "waves_per_cu out of range");
+  if (p->precond != 0 && p->precond != 1) return fail(NCS_E_ARG, "precond must be 0 or 1");
   return NCS_OK;
 }
 
@@ -401,6 +403,7 @@ static int enqueue_solve(ncs_ctx *ctx, const FrameDev *d_frames, int n_frames, b
   a.max_step = p->max_step;
   a.stepm = p->stepm;
   a.fallback = p->fallback;
+  a.precond = p->precond;
   if (total <= 0) return NCS_OK;
   long long waves = (long long)waves_for(ctx, p, pad) * ctx->cus;
   waves = std::max<long long>(1, std::min(waves, total));

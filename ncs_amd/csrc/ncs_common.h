@@ -50,6 +50,7 @@ struct SolveArgs {
   int max_ls;
   float ftol, gtol, xtol, min_step, max_step, stepm;
   int fallback;
+  int precond;                 // spectral initial inverse Hessian (else s.y / y.y)
 };
 
 }  // namespace ncs

@@ -103,10 +103,12 @@ struct MaskTables {
   float qo[4];  // +-2 * odd-odd part, coupling to the partner lane ^ 40
 };
 
-// gnc = K x where K = Hessian of NC (x, gnc in the tile layout).
-template <bool GEN>
-__device__ __forceinline__ void apply_k(const float (&x)[4], const MaskTables &T,
-                                        float (&out)[4]) {
+// out = R^T (w * (R x R^T) [+ qo * partner]) R in the tile layout: the NC
+// Hessian product K x for w = qe (plus the odd-odd coupling when GEN), or
+// the spectral initial inverse Hessian for w = 1 / (kappa + alpha qe).
+template <bool ODD>
+__device__ __forceinline__ void rdft_sandwich(const float (&x)[4], const MaskTables &T,
+                                              const float (&wdiag)[4], float (&out)[4]) {
   f4 z1 = {0.f, 0.f, 0.f, 0.f};
   // (R X)^T = X^T R^T : data as the A operand
   z1 = __builtin_amdgcn_mfma_f32_16x16x4f32(x[0], T.ka[0], z1, 0, 0, 0);
@@ -121,8 +123,8 @@ __device__ __forceinline__ void apply_k(const float (&x)[4], const MaskTables &T
   z = __builtin_amdgcn_mfma_f32_16x16x4f32(T.ka[3], z1[3], z, 0, 0, 0);
   float w[4];
 #pragma unroll
-  for (int r = 0; r < 4; r++) w[r] = T.qe[r] * z[r];
-  if constexpr (GEN) {
+  for (int r = 0; r < 4; r++) w[r] = wdiag[r] * z[r];
+  if constexpr (ODD) {
 #pragma unroll
     for (int r = 0; r < 4; r++) w[r] = fmaf(T.qo[r], __shfl_xor(z[r], 40), w[r]);
   }
@@ -140,6 +142,25 @@ __device__ __forceinline__ void apply_k(const float (&x)[4], const MaskTables &T
   g = __builtin_amdgcn_mfma_f32_16x16x4f32(T.kb[3], e[3], g, 0, 0, 0);
 #pragma unroll
   for (int r = 0; r < 4; r++) out[r] = g[r];
+}
+
+// gnc = K x where K = Hessian of NC (x, gnc in the tile layout).
+template <bool GEN>
+__device__ __forceinline__ void apply_k(const float (&x)[4], const MaskTables &T,
+                                        float (&out)[4]) {
+  rdft_sandwich<GEN>(x, T, T.qe, out);
+}
+
+// x <- H0 x with H0 = R^T diag(1 / (kappa + alpha qe)) R: the inverse of the
+// Hessian with the likelihood part replaced by its tile mean kappa and the
+// mask by its even-even part.
+__device__ __forceinline__ void apply_h0(float (&x)[4], const MaskTables &T, float kappa,
+                                         float alpha) {
+  float w[4];
+#pragma unroll
+  for (int r = 0; r < 4; r++) w[r] = 1.0f / fmaf(alpha, T.qe[r], kappa);
+  float tmp[4] = {x[0], x[1], x[2], x[3]};
+  rdft_sandwich<false>(tmp, T, w, x);
 }
 
 // ---------------------------------------------------------------------------
@@ -371,6 +392,18 @@ __device__ __forceinline__ float ll_lane(const float (&u)[4], const float (&gm)[
 // phi(t) = f(u + t d) - f(u) and of phi'(t), and the likelihood gradient at
 // u + t d.  The NC part is the quadratic t a1 + t^2 a2 / 2 (added by the
 // caller).
+// Per-lane sum of the likelihood curvature c / max(u + gamma, FITMIN)^2.
+__device__ __forceinline__ float curv_lane(const float (&u)[4], const float (&gm)[4],
+                                           const float (&c)[4]) {
+  float s = 0.0f;
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const float rv = rcpf(fmaxf(u[r] + gm[r], kFitMin));
+    s = fmaf(c[r] * rv, rv, s);
+  }
+  return s;
+}
+
 template <bool PAD>
 __device__ __forceinline__ void ll_trial(float t, const float (&u)[4], const float (&dv)[4],
                                          const float (&gm)[4], const float (&c)[4],
@@ -501,13 +534,15 @@ void solve_kernel(SolveArgs A) {
 
     int status = NCS_REACHED_MAXITERS, iters = 0, evals = 1;
     bool gnc_exact = true;
-    float gg, uu;
+    float gg, uu, kappa;
     {
-      float red[2] = {g[0] * g[0] + g[1] * g[1] + (g[2] * g[2] + g[3] * g[3]),
-                      u[0] * u[0] + u[1] * u[1] + (u[2] * u[2] + u[3] * u[3])};
-      wave_sum<2>(red);
+      float red[3] = {g[0] * g[0] + g[1] * g[1] + (g[2] * g[2] + g[3] * g[3]),
+                      u[0] * u[0] + u[1] * u[1] + (u[2] * u[2] + u[3] * u[3]),
+                      curv_lane(u, gm, c)};
+      wave_sum<3>(red);
       gg = red[0];
       uu = red[1];
+      kappa = red[2] * (1.0f / 256.0f);
     }
     bool done = sqrtf(gg) <= A.eps * fmaxf(1.0f, sqrtf(uu));
     if (done) status = NCS_SUCCESS;
@@ -518,6 +553,10 @@ void solve_kernel(SolveArgs A) {
 #pragma unroll
     for (int r = 0; r < 4; r++) dvec[r] = -g[r];
     float step = rsqrtf(gg);
+    if (A.precond && !done) {
+      apply_h0(dvec, T, kappa, alpha);
+      step = 1.0f;
+    }
     float ys_last = 1.0f, yy_last = 1.0f;
 
     for (int k = 1; !done && k <= A.max_it; k++) {
@@ -619,16 +658,18 @@ void solve_kernel(SolveArgs A) {
           gnew[r] = fmaf(alpha, gncn[r], gt[r]);
           y[r] = gnew[r] - g[r];
         }
-        float red[4] = {
+        float red[5] = {
             y[0] * s[0] + y[1] * s[1] + (y[2] * s[2] + y[3] * s[3]),
             y[0] * y[0] + y[1] * y[1] + (y[2] * y[2] + y[3] * y[3]),
             gnew[0] * gnew[0] + gnew[1] * gnew[1] + (gnew[2] * gnew[2] + gnew[3] * gnew[3]),
-            ut[0] * ut[0] + ut[1] * ut[1] + (ut[2] * ut[2] + ut[3] * ut[3])};
-        wave_sum<4>(red);
+            ut[0] * ut[0] + ut[1] * ut[1] + (ut[2] * ut[2] + ut[3] * ut[3]),
+            A.precond ? curv_lane(ut, gm, c) : 0.0f};
+        wave_sum<5>(red);
         ys = red[0];
         yy = red[1];
         gg = red[2];
         uu = red[3];
+        kappa = red[4] * (1.0f / 256.0f);
         done = sqrtf(gg) <= A.eps * fmaxf(1.0f, sqrtf(uu));
         if (!done || gnc_exact) break;
         // confirm convergence with an exact NC gradient
@@ -675,9 +716,13 @@ void solve_kernel(SolveArgs A) {
           dvec[3] = fmaf(-al[j], yv.w, dvec[3]);
         }
       }
-      const float hscale = (nh > 0) ? ys_last / yy_last : rsqrtf(gg);
+      if (A.precond) {
+        apply_h0(dvec, T, kappa, alpha);
+      } else {  // liblbfgs: H0 = (s.y / y.y) I
+        const float hscale = (nh > 0) ? ys_last / yy_last : rsqrtf(gg);
 #pragma unroll
-      for (int r = 0; r < 4; r++) dvec[r] *= hscale;
+        for (int r = 0; r < 4; r++) dvec[r] *= hscale;
+      }
 #pragma unroll
       for (int j = M - 1; j >= 0; j--) {
         if (j < nh) {

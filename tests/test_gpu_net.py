@@ -131,20 +131,23 @@ def test_training_lowers_the_loss_and_saves_iterations():
     u0 = warmstart.predict_u0(params, dt, gt_).detach()
     ctx = _lib.context(0)
     n = dt.shape[0]
-    res = {}
-    for name, start in (("plain", None), ("learned", u0)):
-        out = torch.zeros_like(dt)
-        it = torch.zeros(n, dtype=torch.int32, device="cuda")
-        st = torch.zeros_like(it)
-        cs = torch.zeros(n, device="cuda")
-        ctx.solve_tiles_device(dt.data_ptr(), gt_.data_ptr(),
-                               None if start is None else start.contiguous().data_ptr(),
-                               out.data_ptr(), it.data_ptr(), st.data_ptr(), cs.data_ptr(), n,
-                               0.1, _lib.default_params(), torch.cuda.current_stream().cuda_stream)
-        torch.cuda.synchronize()
-        res[name] = (it.float().mean().item(), cs.cpu().numpy())
-    rel = numpy.abs(res["learned"][1] - res["plain"][1]) / numpy.abs(res["plain"][1])
-    _say("iterations plain %.2f learned %.2f, cost max rel %.2e"
-         % (res["plain"][0], res["learned"][0], rel.max()))
-    assert rel.max() <= 1e-5
-    assert res["learned"][0] < res["plain"][0]
+    for precond in (0, 1):
+        res = {}
+        for name, start in (("plain", None), ("learned", u0)):
+            out = torch.zeros_like(dt)
+            it = torch.zeros(n, dtype=torch.int32, device="cuda")
+            st = torch.zeros_like(it)
+            cs = torch.zeros(n, device="cuda")
+            ctx.solve_tiles_device(dt.data_ptr(), gt_.data_ptr(),
+                                   None if start is None else start.contiguous().data_ptr(),
+                                   out.data_ptr(), it.data_ptr(), st.data_ptr(), cs.data_ptr(), n,
+                                   0.1, _lib.default_params(precond=precond),
+                                   torch.cuda.current_stream().cuda_stream)
+            torch.cuda.synchronize()
+            res[name] = (it.float().mean().item(), cs.cpu().numpy())
+        rel = numpy.abs(res["learned"][1] - res["plain"][1]) / numpy.abs(res["plain"][1])
+        _say("precond %d: iterations plain %.2f learned %.2f, cost max rel %.2e"
+             % (precond, res["plain"][0], res["learned"][0], rel.max()))
+        assert rel.max() <= 1e-5
+        if precond == 0:  # the reference's L-BFGS (scalar initial Hessian)
+            assert res["learned"][0] < res["plain"][0]

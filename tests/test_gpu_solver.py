@@ -70,15 +70,19 @@ def test_solver_reference_test_case(gpu_ctx, mask):
         assert nd < 1e-2, (i, nd)
 
 
+@pytest.mark.parametrize("precond", [0, 1])
 @pytest.mark.parametrize("history", [1, 3, 5, 6, 8])
-def test_solver_synthetic_tiles_every_history(gpu_ctx, history):
-    _say("synthetic tiles, history", history)
+def test_solver_synthetic_tiles_every_history(gpu_ctx, history, precond):
+    """Both initial inverse Hessians (liblbfgs's scalar s.y/y.y and the
+    spectral one) reach the oracle's minimiser for every history length."""
+    _say("synthetic tiles, history", history, "precond", precond)
     fr = sim.make_frame(7, (128, 128))
     rng = numpy.random.default_rng(8)
     d, g = _tiles_from_frame(fr, 96, rng)
     mf = masks.to_fft_order(masks.otf_mask()).astype(numpy.float32)
     gpu_ctx.set_otf_mask(mf)
-    u, it, st, cost = gpu_ctx.solve_tiles(d, g, 0.1, _lib.default_params(history=history))
+    u, it, st, cost = gpu_ctx.solve_tiles(d, g, 0.1,
+                                          _lib.default_params(history=history, precond=precond))
     ref = oracle.solve_tiles(d, g, mf, 0.1, f32=True, param=oracle.default_param(m=history))
     f64, gn, un = oracle.eval_cost_f32(u, d, g, mf, 0.1)
     rel = (f64 - ref["cost"]) / numpy.abs(ref["cost"])
