@@ -57,7 +57,8 @@ extern "C" {
 
 typedef struct {
   int max_iterations;  /* L-BFGS iteration cap (ncs.cl:61 MAXITERS = 200) */
-  int history;         /* correction pairs m, 1..8 (lbfgs.h default 6, ncs.cl 8) */
+  int history;         /* correction pairs m, 1..8 (lbfgs.h default 6, ncs.cl 8);
+                          0 = the autotuned depth (5 until ncs_hip_autotune ran) */
   float epsilon;       /* stop when |g| <= epsilon * max(1, |u|); 0 disables */
   int linesearch;      /* NCS_LS_* */
   int max_linesearch;  /* trials per iteration */
@@ -69,11 +70,14 @@ typedef struct {
   float stepm;         /* Armijo backtracking factor */
   int start;           /* NCS_START_* */
   int fallback;        /* warm starts: keep u0 = data where f(data) <= f(u0) */
-  int waves_per_cu;    /* persistent grid size; 0 = autotuned */
+  int waves_per_cu;    /* persistent grid size; 0 = autotuned (else the
+                          occupancy limit) */
   int precond;         /* 1: initial inverse Hessian H0 = R^T diag(1/(kappa +
                           alpha*2q)) R in the real DFT basis R, kappa = tile
                           mean of (d+gamma)/(u+gamma)^2 (default); 0: the
                           scalar s.y/y.y of liblbfgs and ncs.cl:1123 */
+  int waves_per_group; /* tiles (waves) per workgroup, 1..4; 0 = autotuned
+                          (else 1) */
 } ncs_params;
 
 typedef struct {
@@ -92,7 +96,8 @@ typedef struct ncs_ctx ncs_ctx;
 
 /* Fill defaults: max_iterations 200, history 5, epsilon 1e-5, More-Thuente
  * (ftol 1e-4, gtol 0.9, xtol 1e-7, max_linesearch 20, steps 1e-20..1e20),
- * Armijo stepm 0.5, plain start, fallback on, autotuned grid, spectral H0. */
+ * Armijo stepm 0.5, plain start, fallback on, autotuned grid and workgroup
+ * size, spectral H0. */
 void ncs_hip_default_params(ncs_params *p);
 
 int ncs_hip_create(int device, ncs_ctx **out);
@@ -197,6 +202,84 @@ int ncs_hip_net_backward(ncs_ctx *ctx, const float *params, const float *x,
                          const void *acts, const float *grad_out,
                          float *grad_params, float *grad_x, int n,
                          void *stream);
+
+/* ---- autotuner (no reference counterpart) ---- */
+
+typedef struct {
+  int history;         /* chosen L-BFGS depth (the given one unless tuned) */
+  int waves_per_cu;    /* chosen persistent grid, waves per CU */
+  int waves_per_group; /* chosen tiles (waves) per workgroup */
+  int lds_waves_per_cu;/* occupancy limit of the chosen depth (LDS/VGPR) */
+  float ms;            /* time of the chosen schedule on the sample */
+  float ms_default;    /* time of the untuned schedule on the sample */
+  int candidates;      /* schedules timed */
+} ncs_tune_result;
+
+/* Times the persistent solve on a sample of the workload (device frames, as
+ * for ncs_hip_reduce_frames_device; nothing is written to their outputs) over
+ * candidate schedules -- waves per CU up to the occupancy that the history's
+ * LDS rings allow, 1/2/4 tiles per workgroup and, if tune_history is set,
+ * history depths 3..8 -- and keeps the fastest in the context's cache for the
+ * mask kind and geometry of the sample.  Later calls whose params leave
+ * waves_per_cu, waves_per_group or history at 0 use it.  Synchronous. */
+int ncs_hip_autotune(ncs_ctx *ctx, const ncs_frame *frames, int n_frames,
+                     const ncs_params *base, int tune_history,
+                     ncs_tune_result *result);
+
+/* ---- captured streaming pipeline (no reference counterpart) ----
+ *
+ * A fixed batch shape is captured once as a hipGraph: upload of every frame
+ * from host memory -> camera conversion (raw input) -> learned start (if
+ * params->start is NCS_START_LEARNED) -> work-queue reset -> persistent solve
+ * with the stitch fused -> download of every corrected frame (and, if asked,
+ * of the per-tile results).  Each replay may point the copies at other host
+ * buffers; pinned buffers (hipHostMalloc) make the copies asynchronous.  A
+ * pipeline runs one replay at a time (ncs_hip_pipeline_launch waits for the
+ * previous one); two pipelines overlap one frame's copies with another's
+ * solve.  Gamma (or the camera maps) are uploaded once per pipeline. */
+
+#define NCS_INPUT_PHOTONS_F32 0 /* data in photo-electrons, float32 [H][W] */
+#define NCS_INPUT_ADU_U16 1     /* raw camera ADU, uint16 [H][W]; photons =
+                                   (ADU - offset) / gain on the device, values
+                                   <= 0 set to 1e-6 (denoisetools.py:218-224) */
+
+typedef struct {
+  int height, width;
+  float alpha;
+  int geometry; /* NCS_GEOM_* */
+} ncs_pipeline_frame;
+
+typedef struct ncs_pipeline ncs_pipeline;
+
+/* The pipeline keeps a pointer to ctx, which must outlive it; the context's
+ * mask and network are read at every replay (a change of mask kind, even to
+ * general, needs a new pipeline). */
+int ncs_hip_pipeline_create(ncs_ctx *ctx, const ncs_pipeline_frame *frames,
+                            int n_frames, const ncs_params *params,
+                            int input_format, int keep_tile_results,
+                            ncs_pipeline **out);
+void ncs_hip_pipeline_destroy(ncs_pipeline *pl);
+/* Gamma of frame i (host float32 [H][W], photons^2); synchronous. */
+int ncs_hip_pipeline_set_gamma(ncs_pipeline *pl, int frame, const float *gamma);
+/* Camera maps of frame i for NCS_INPUT_ADU_U16 (host float32 [H][W]): offset
+ * (ADU), gain (ADU/e-), read variance (ADU^2); also sets gamma = variance /
+ * gain^2 (denoisetools.py:266).  Synchronous. */
+int ncs_hip_pipeline_set_camera(ncs_pipeline *pl, int frame, const float *offset,
+                                const float *gain, const float *variance);
+/* Enqueue one replay on `stream` (0: the pipeline's own stream): inputs[i]
+ * is frame i's host input, outputs[i] its host float32 output [H][W].  The
+ * first launch needs both arrays; later ones may pass NULL to keep the
+ * previous pointers. */
+int ncs_hip_pipeline_launch(ncs_pipeline *pl, const void *const *inputs,
+                            float *const *outputs, void *stream);
+/* Wait for the last replay (its outputs are then in host memory). */
+int ncs_hip_pipeline_wait(ncs_pipeline *pl);
+/* Per-tile results of the last replay (after wait), as for
+ * ncs_hip_reduce_frames; needs keep_tile_results.  Any may be NULL. */
+int ncs_hip_pipeline_tile_results(ncs_pipeline *pl, int *iterations, int *status,
+                                  float *cost);
+/* Tiles of one replay. */
+long long ncs_hip_pipeline_tiles(const ncs_pipeline *pl);
 
 #ifdef __cplusplus
 }

@@ -29,6 +29,8 @@ START_PLAIN = 0
 START_TILES = 1
 START_LEARNED = 2
 NET_PARAMS = 19393
+INPUT_PHOTONS_F32 = 0
+INPUT_ADU_U16 = 1
 
 
 class NCSHipError(RuntimeError):
@@ -51,7 +53,8 @@ class Params(ctypes.Structure):
                 ("start", ctypes.c_int),
                 ("fallback", ctypes.c_int),
                 ("waves_per_cu", ctypes.c_int),
-                ("precond", ctypes.c_int)]
+                ("precond", ctypes.c_int),
+                ("waves_per_group", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -67,6 +70,28 @@ class Frame(ctypes.Structure):
                 ("data_stride", ctypes.c_int),
                 ("gamma_stride", ctypes.c_int),
                 ("out_stride", ctypes.c_int),
+                ("alpha", ctypes.c_float),
+                ("geometry", ctypes.c_int)]
+
+
+class TuneResult(ctypes.Structure):
+    """ncs_tune_result (include/ncs_hip.h)."""
+    _fields_ = [("history", ctypes.c_int),
+                ("waves_per_cu", ctypes.c_int),
+                ("waves_per_group", ctypes.c_int),
+                ("lds_waves_per_cu", ctypes.c_int),
+                ("ms", ctypes.c_float),
+                ("ms_default", ctypes.c_float),
+                ("candidates", ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class PipelineFrame(ctypes.Structure):
+    """ncs_pipeline_frame (include/ncs_hip.h)."""
+    _fields_ = [("height", ctypes.c_int),
+                ("width", ctypes.c_int),
                 ("alpha", ctypes.c_float),
                 ("geometry", ctypes.c_int)]
 
@@ -95,6 +120,16 @@ SIGNATURES = {
     "ncs_hip_net_forward": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _vp]),
     "ncs_hip_net_backward": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp]),
     "ncsReduceNoise": (None, [_vp, _vp, _vp, _vp, ctypes.c_double, _i, _i, _i]),
+    "ncs_hip_autotune": (_i, [_vp, _F, _i, _P, _i, ctypes.POINTER(TuneResult)]),
+    "ncs_hip_pipeline_create": (_i, [_vp, ctypes.POINTER(PipelineFrame), _i, _P, _i, _i,
+                                     ctypes.POINTER(_vp)]),
+    "ncs_hip_pipeline_destroy": (None, [_vp]),
+    "ncs_hip_pipeline_set_gamma": (_i, [_vp, _i, _vp]),
+    "ncs_hip_pipeline_set_camera": (_i, [_vp, _i, _vp, _vp, _vp]),
+    "ncs_hip_pipeline_launch": (_i, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp]),
+    "ncs_hip_pipeline_wait": (_i, [_vp]),
+    "ncs_hip_pipeline_tile_results": (_i, [_vp, _vp, _vp, _vp]),
+    "ncs_hip_pipeline_tiles": (_ll, [_vp]),
 }
 
 _lib = None
@@ -280,6 +315,17 @@ class Context(object):
                                                          len(frames), u0_ptr, stream_ptr),
               "ncs_hip_net_predict_frames_device")
 
+    def autotune(self, frames, params=None, tune_history=False):
+        """Time candidate schedules on device frames (dicts as for
+        reduce_frames_device) and cache the fastest; returns the result dict."""
+        res = TuneResult()
+        p = params if params is not None else default_params()
+        check(self.lib.ncs_hip_autotune(self.handle, frame_array(frames), len(frames),
+                                        ctypes.byref(p), int(bool(tune_history)),
+                                        ctypes.byref(res)),
+              "ncs_hip_autotune")
+        return res.as_dict()
+
     def reduce_frames_device(self, frames, params, u0_ptr=None, iters_ptr=None,
                              status_ptr=None, cost_ptr=None, stream_ptr=None):
         """frames: list of dicts(data, gamma, out, height, width, alpha, geometry,
@@ -320,3 +366,88 @@ def context(device=0):
         with _lock:
             _contexts[device] = ctx
     return ctx
+
+
+class Pipeline(object):
+    """A captured upload -> solve -> download graph for a fixed batch shape
+    (ncs_hip_pipeline_*).  shapes: list of (height, width); alphas: one per
+    frame; inputs and outputs are host numpy arrays (pinned memory makes the
+    copies asynchronous)."""
+
+    def __init__(self, ctx, shapes, alphas, params=None, geometry=GEOM_REFERENCE,
+                 input_format=INPUT_PHOTONS_F32, keep_tile_results=False):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        n = len(shapes)
+        fr = (PipelineFrame * n)()
+        for i, (h, w) in enumerate(shapes):
+            fr[i].height, fr[i].width = int(h), int(w)
+            fr[i].alpha = float(alphas[i])
+            fr[i].geometry = int(geometry[i] if hasattr(geometry, "__len__") else geometry)
+        p = params if params is not None else default_params()
+        h = ctypes.c_void_p()
+        check(self.lib.ncs_hip_pipeline_create(ctx.handle, fr, n, ctypes.byref(p),
+                                               int(input_format), int(bool(keep_tile_results)),
+                                               ctypes.byref(h)),
+              "ncs_hip_pipeline_create")
+        self.handle = h
+        self.shapes = [tuple(int(v) for v in s) for s in shapes]
+        self.input_dtype = numpy.uint16 if input_format == INPUT_ADU_U16 else numpy.float32
+        self.n_tiles = int(self.lib.ncs_hip_pipeline_tiles(h))
+        self._keep = []
+
+    def close(self):
+        if self.handle is not None:
+            self.lib.ncs_hip_pipeline_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check_frame(self, i, a, dtype):
+        if a.shape != self.shapes[i] or a.dtype != dtype or not a.flags.c_contiguous:
+            raise ValueError("frame %d: expected C-contiguous %s of shape %s"
+                             % (i, numpy.dtype(dtype).name, self.shapes[i]))
+
+    def set_gamma(self, i, gamma):
+        g = numpy.ascontiguousarray(gamma, dtype=numpy.float32)
+        self._check_frame(i, g, numpy.float32)
+        check(self.lib.ncs_hip_pipeline_set_gamma(self.handle, int(i), _ptr(g)),
+              "ncs_hip_pipeline_set_gamma")
+
+    def set_camera(self, i, offset, gain, variance):
+        maps = [numpy.ascontiguousarray(m, dtype=numpy.float32) for m in (offset, gain, variance)]
+        for m in maps:
+            self._check_frame(i, m, numpy.float32)
+        check(self.lib.ncs_hip_pipeline_set_camera(self.handle, int(i), *[_ptr(m) for m in maps]),
+              "ncs_hip_pipeline_set_camera")
+
+    def launch(self, inputs=None, outputs=None, stream_ptr=None):
+        """Enqueue one replay.  The arrays must stay alive until wait()."""
+        n = len(self.shapes)
+        ins = outs = None
+        if inputs is not None:
+            for i, a in enumerate(inputs):
+                self._check_frame(i, a, self.input_dtype)
+            ins = (ctypes.c_void_p * n)(*[a.ctypes.data for a in inputs])
+        if outputs is not None:
+            for i, a in enumerate(outputs):
+                self._check_frame(i, a, numpy.float32)
+            outs = (ctypes.c_void_p * n)(*[a.ctypes.data for a in outputs])
+        check(self.lib.ncs_hip_pipeline_launch(self.handle, ins, outs, stream_ptr),
+              "ncs_hip_pipeline_launch")
+        self._keep = [inputs, outputs]
+
+    def wait(self):
+        check(self.lib.ncs_hip_pipeline_wait(self.handle), "ncs_hip_pipeline_wait")
+
+    def tile_results(self):
+        it = numpy.zeros(self.n_tiles, numpy.int32)
+        st = numpy.zeros(self.n_tiles, numpy.int32)
+        co = numpy.zeros(self.n_tiles, numpy.float32)
+        check(self.lib.ncs_hip_pipeline_tile_results(self.handle, _ptr(it), _ptr(st), _ptr(co)),
+              "ncs_hip_pipeline_tile_results")
+        return it, st, co

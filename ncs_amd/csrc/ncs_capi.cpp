@@ -13,9 +13,13 @@
 #include "ncs_common.h"
 
 namespace ncs {
-int solve_waves_per_cu(int m, bool general, bool pad);
-hipError_t launch_solve(int m, bool general, bool pad, unsigned int waves, hipStream_t s,
-                        const SolveArgs &a);
+int solve_waves_per_cu(int m, bool general, bool pad, int wpg);
+hipError_t launch_solve(int m, bool general, bool pad, unsigned int waves, int wpg,
+                        hipStream_t s, const SolveArgs &a);
+hipError_t launch_adu_to_photons(const uint16_t *adu, const float *offset, const float *gain,
+                                 float *out, long long n, hipStream_t s);
+hipError_t launch_gamma_from_camera(const float *var, const float *gain, float *gamma,
+                                    long long n, hipStream_t s);
 hipError_t net_pack(const float *P, void *packs, hipStream_t s);
 size_t net_packs_bytes();
 hipError_t net_forward(const float *x, float *y, void *acts, const void *packs, const float *P,
@@ -77,7 +81,12 @@ struct DevBuf {
   template <class T> T *as() const { return reinterpret_cast<T *>(p); }
 };
 
-struct GraphRec;
+// A persistent-solve schedule: history depth, grid and workgroup size.
+struct Schedule {
+  int history = 5;
+  int waves_per_cu = 0;  // 0: the occupancy limit
+  int wpg = 1;
+};
 
 struct ncs_ctx {
   int device = 0;
@@ -88,9 +97,8 @@ struct ncs_ctx {
   bool has_mask = false;
   bool general = false;
   float mask_fft[256];
-  std::map<int, int> tuned_waves;  // history -> waves per CU
-  std::map<int, std::unique_ptr<GraphRec>> graphs;
-  int next_graph = 1;
+  // autotuned schedules, keyed by (general mask, padded geometry)
+  std::map<std::pair<bool, bool>, Schedule> tuned;
   // last uploaded frame descriptors (pinned staging + completion event)
   std::vector<FrameDev> last_frames;
   bool frames_valid = false;
@@ -103,16 +111,6 @@ struct ncs_ctx {
   bool has_net = false;
 };
 
-struct GraphRec {
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  std::vector<FrameDev> fdev;
-  float *h_data = nullptr, *h_gamma = nullptr, *h_out = nullptr, *h_cost = nullptr;
-  int *h_iters = nullptr, *h_status = nullptr;
-  DevBuf d_data, d_gamma, d_out, d_frames, d_queue, d_iters, d_status, d_cost;
-  size_t n_pix = 0, n_gamma = 0;
-  long long n_tiles = 0;
-};
 
 static hipError_t set_device(ncs_ctx *ctx) { return hipSetDevice(ctx->device); }
 
@@ -132,6 +130,7 @@ extern "C" void ncs_hip_default_params(ncs_params *p) {
   p->fallback = 1;
   p->waves_per_cu = 0;
   p->precond = 1;
+  p->waves_per_group = 0;
 }
 
 extern "C" const char *ncs_hip_last_error(void) { return g_last_error.c_str(); }
@@ -156,26 +155,10 @@ extern "C" int ncs_hip_create(int device, ncs_ctx **out) {
   return NCS_OK;
 }
 
-static void destroy_graph(GraphRec *g) {
-  if (g->exec) (void)hipGraphExecDestroy(g->exec);
-  if (g->graph) (void)hipGraphDestroy(g->graph);
-  if (g->h_data) (void)hipHostFree(g->h_data);
-  if (g->h_gamma) (void)hipHostFree(g->h_gamma);
-  if (g->h_out) (void)hipHostFree(g->h_out);
-  if (g->h_iters) (void)hipHostFree(g->h_iters);
-  if (g->h_status) (void)hipHostFree(g->h_status);
-  if (g->h_cost) (void)hipHostFree(g->h_cost);
-  g->d_data.release(); g->d_gamma.release(); g->d_out.release();
-  g->d_frames.release(); g->d_queue.release(); g->d_iters.release();
-  g->d_status.release(); g->d_cost.release();
-}
-
 extern "C" void ncs_hip_destroy(ncs_ctx *ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-  for (auto &kv : ctx->graphs) destroy_graph(kv.second.get());
-  ctx->graphs.clear();
   DevBuf *bufs[] = {&ctx->tables, &ctx->frames, &ctx->queue, &ctx->data, &ctx->gamma,
                     &ctx->out, &ctx->iters, &ctx->status, &ctx->cost, &ctx->tiles_u,
                     &ctx->net_params, &ctx->net_packs, &ctx->net_tmp_packs, &ctx->u0_buf};
@@ -273,7 +256,8 @@ extern "C" long long ncs_hip_tile_count(int height, int width, int geometry) {
 
 static int check_params(const ncs_params *p) {
   if (!p) return fail(NCS_E_ARG, "params is NULL");
-  if (p->history < 1 || p->history > 8) return fail(NCS_E_ARG, "history must be 1..8");
+  if (p->history < 0 || p->history > 8)
+    return fail(NCS_E_ARG, "history must be 1..8 (or 0 for the autotuned depth)");
   if (p->max_iterations < 0 || p->max_iterations > 100000)
     return fail(NCS_E_ARG, "max_iterations out of range");
   if (p->max_linesearch < 1 || p->max_linesearch > 1000)
@@ -291,6 +275,8 @@ static int check_params(const ncs_params *p) {
   if (!(p->xtol >= 0.0f)) return fail(NCS_E_ARG, "xtol must be >= 0");
   if (p->waves_per_cu < 0 || p->waves_per_cu > 64) return fail(NCS_E_ARG, "waves_per_cu out of range");
   if (p->precond != 0 && p->precond != 1) return fail(NCS_E_ARG, "precond must be 0 or 1");
+  if (p->waves_per_group < 0 || p->waves_per_group > kMaxWavesPerGroup)
+    return fail(NCS_E_ARG, "waves_per_group must be 0..4");
   return NCS_OK;
 }
 
@@ -365,21 +351,30 @@ static bool any_pad(const ncs_frame *frames, int n) {
   return false;
 }
 
-static int waves_for(ncs_ctx *ctx, const ncs_params *p, bool pad) {
-  if (p->waves_per_cu > 0) return p->waves_per_cu;
-  auto it = ctx->tuned_waves.find(p->history);
-  if (it != ctx->tuned_waves.end()) return it->second;
-  int w = solve_waves_per_cu(p->history, ctx->general, pad);
-  return w > 0 ? w : 8;
+// The schedule of a solve: explicit params win, then the autotuned schedule
+// of this mask kind and geometry, then history 5, one wave per workgroup and
+// the occupancy limit.
+static Schedule resolve_schedule(const ncs_ctx *ctx, const ncs_params *p, bool pad) {
+  Schedule d;
+  auto it = ctx->tuned.find({ctx->general, pad});
+  if (it != ctx->tuned.end()) d = it->second;
+  Schedule s;
+  s.history = p->history > 0 ? p->history : d.history;
+  s.wpg = p->waves_per_group > 0 ? p->waves_per_group : d.wpg;
+  // a tuned grid only applies to the depth and workgroup size it was tuned for
+  const bool same = s.history == d.history && s.wpg == d.wpg;
+  s.waves_per_cu = p->waves_per_cu > 0 ? p->waves_per_cu : (same ? d.waves_per_cu : 0);
+  if (s.waves_per_cu <= 0) {
+    int w = solve_waves_per_cu(s.history, ctx->general, pad, s.wpg);
+    s.waves_per_cu = w > 0 ? w : 8;
+  }
+  return s;
 }
 
-// Enqueue one persistent solve over device frames already described in
-// d_frames (device copy of fd).
-static int enqueue_solve(ncs_ctx *ctx, const FrameDev *d_frames, int n_frames, bool pad,
-                         long long total, const ncs_params *p, const float *u0,
-                         float *u_tiles, int *iters, int *status, float *cost,
-                         unsigned int *queue, hipStream_t s) {
-  if (!ctx->has_mask) return fail(NCS_E_NOMASK, "no OTF mask set");
+static SolveArgs make_args(const ncs_ctx *ctx, const FrameDev *d_frames, int n_frames,
+                           long long total, const ncs_params *p, const float *u0,
+                           float *u_tiles, int *iters, int *status, float *cost,
+                           unsigned int *queue) {
   SolveArgs a;
   a.frames = d_frames;
   a.n_frames = n_frames;
@@ -404,10 +399,27 @@ static int enqueue_solve(ncs_ctx *ctx, const FrameDev *d_frames, int n_frames, b
   a.stepm = p->stepm;
   a.fallback = p->fallback;
   a.precond = p->precond;
-  if (total <= 0) return NCS_OK;
-  long long waves = (long long)waves_for(ctx, p, pad) * ctx->cus;
+  return a;
+}
+
+static hipError_t launch_schedule(const ncs_ctx *ctx, const Schedule &sc, bool pad,
+                                  long long total, hipStream_t s, const SolveArgs &a) {
+  long long waves = (long long)sc.waves_per_cu * ctx->cus;
   waves = std::max<long long>(1, std::min(waves, total));
-  HIPCHK(launch_solve(p->history, ctx->general, pad, (unsigned int)waves, s, a));
+  return launch_solve(sc.history, ctx->general, pad, (unsigned int)waves, sc.wpg, s, a);
+}
+
+// Enqueue one persistent solve over device frames already described in
+// d_frames (device copy of fd).
+static int enqueue_solve(ncs_ctx *ctx, const FrameDev *d_frames, int n_frames, bool pad,
+                         long long total, const ncs_params *p, const float *u0,
+                         float *u_tiles, int *iters, int *status, float *cost,
+                         unsigned int *queue, hipStream_t s) {
+  if (!ctx->has_mask) return fail(NCS_E_NOMASK, "no OTF mask set");
+  if (total <= 0) return NCS_OK;
+  SolveArgs a = make_args(ctx, d_frames, n_frames, total, p, u0, u_tiles, iters, status, cost,
+                          queue);
+  HIPCHK(launch_schedule(ctx, resolve_schedule(ctx, p, pad), pad, total, s, a));
   return NCS_OK;
 }
 
@@ -723,4 +735,443 @@ extern "C" void ncsReduceNoise(double *ncs_image, double *image, double *gamma,
   if (ncs_hip_reduce_frames(g_default_ctx, &f, 1, &p, nullptr, nullptr, nullptr) != NCS_OK)
     return;
   for (size_t i = 0; i < n; i++) ncs_image[i] = o[i];
+}
+
+// ---------------------------------------------------------------------------
+// autotuner
+// ---------------------------------------------------------------------------
+
+extern "C" int ncs_hip_autotune(ncs_ctx *ctx, const ncs_frame *frames, int n_frames,
+                                const ncs_params *base, int tune_history,
+                                ncs_tune_result *res) {
+  if (!ctx || !frames || n_frames <= 0 || !res) return fail(NCS_E_ARG, "bad arguments");
+  int rc = check_params(base);
+  if (rc) return rc;
+  if (!ctx->has_mask) return fail(NCS_E_NOMASK, "no OTF mask set");
+  if (base->start != NCS_START_PLAIN) return fail(NCS_E_ARG, "the autotuner times the plain start");
+  std::vector<FrameDev> fd;
+  long long total = 0;
+  rc = build_frames(frames, n_frames, fd, total);
+  if (rc) return rc;
+  if (total <= 0) return fail(NCS_E_ARG, "empty sample");
+  for (FrameDev &f : fd) f.out = nullptr;  // time the solve, write nothing
+  const bool pad = any_pad(frames, n_frames);
+  HIPCHK(set_device(ctx));
+  hipStream_t s = ctx->stream;
+  DevBuf dfr, q;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  auto cleanup = [&]() {
+    dfr.release();
+    q.release();
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+  };
+  hipError_t e = dfr.ensure(fd.size() * sizeof(FrameDev));
+  if (e == hipSuccess) e = q.ensure(256);
+  if (e == hipSuccess)
+    e = hipMemcpy(dfr.p, fd.data(), fd.size() * sizeof(FrameDev), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e != hipSuccess) {
+    cleanup();
+    return fail(NCS_E_HIP, std::string("autotune setup: ") + hipGetErrorString(e));
+  }
+  const SolveArgs a = make_args(ctx, dfr.as<FrameDev>(), n_frames, total, base, nullptr,
+                                nullptr, nullptr, nullptr, nullptr, q.as<unsigned int>());
+  // one untimed run (code load, caches), then the best of two
+  auto time_one = [&](const Schedule &sc, float *ms) -> hipError_t {
+    hipError_t r = launch_schedule(ctx, sc, pad, total, s, a);
+    float best = 3.0e38f;
+    for (int rep = 0; rep < 2 && r == hipSuccess; rep++) {
+      r = hipEventRecord(e0, s);
+      if (r == hipSuccess) r = launch_schedule(ctx, sc, pad, total, s, a);
+      if (r == hipSuccess) r = hipEventRecord(e1, s);
+      if (r == hipSuccess) r = hipEventSynchronize(e1);
+      float t = 0.0f;
+      if (r == hipSuccess) r = hipEventElapsedTime(&t, e0, e1);
+      best = std::min(best, t);
+    }
+    *ms = best;
+    return r;
+  };
+  Schedule def;
+  def.history = base->history > 0 ? base->history : 5;
+  def.wpg = 1;
+  def.waves_per_cu = std::max(1, solve_waves_per_cu(def.history, ctx->general, pad, 1));
+  float ms_def = 0.0f;
+  e = time_one(def, &ms_def);
+  Schedule best = def;
+  float ms_best = ms_def;
+  int tried = 1, best_occ = def.waves_per_cu;
+  std::vector<int> depths;
+  if (tune_history) depths = {3, 4, 5, 6, 8};
+  else depths = {def.history};
+  const int groups[] = {1, 2, 4};
+  for (int m : depths) {
+    for (int wpg : groups) {
+      if (e != hipSuccess) break;
+      const int occ = solve_waves_per_cu(m, ctx->general, pad, wpg);
+      if (occ <= 0) continue;
+      std::vector<int> grid = {occ, std::max(wpg, (occ * 3 / 4) / wpg * wpg),
+                               std::max(wpg, (occ / 2) / wpg * wpg)};
+      std::sort(grid.begin(), grid.end());
+      grid.erase(std::unique(grid.begin(), grid.end()), grid.end());
+      for (int w : grid) {
+        Schedule sc;
+        sc.history = m;
+        sc.wpg = wpg;
+        sc.waves_per_cu = w;
+        if (sc.history == def.history && sc.wpg == def.wpg && sc.waves_per_cu == def.waves_per_cu)
+          continue;
+        float ms = 0.0f;
+        e = time_one(sc, &ms);
+        if (e != hipSuccess) break;
+        tried++;
+        if (ms < ms_best) {
+          ms_best = ms;
+          best = sc;
+          best_occ = occ;
+        }
+      }
+    }
+  }
+  cleanup();
+  HIPCHK(e);
+  if (!tune_history) best.history = def.history;
+  ctx->tuned[{ctx->general, pad}] = best;
+  res->history = best.history;
+  res->waves_per_cu = best.waves_per_cu;
+  res->waves_per_group = best.wpg;
+  res->lds_waves_per_cu = best_occ;
+  res->ms = ms_best;
+  res->ms_default = ms_def;
+  res->candidates = tried;
+  return NCS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// captured streaming pipeline
+// ---------------------------------------------------------------------------
+
+struct ncs_pipeline {
+  ncs_ctx *ctx = nullptr;
+  int n = 0;
+  std::vector<ncs_pipeline_frame> shape;
+  ncs_params params;
+  int input = NCS_INPUT_PHOTONS_F32;
+  size_t elem = 4;              // bytes per input pixel
+  std::vector<size_t> off;      // first pixel of each frame in the packed buffers
+  size_t pix = 0;
+  long long total = 0;
+  bool pad = false, general = false, keep = false, learned = false;
+  Schedule sched;
+  DevBuf raw, data, gamma, out, offset, gain, frames, queue, iters, status, cost, u0;
+  int *h_iters = nullptr, *h_status = nullptr;
+  float *h_cost = nullptr;
+  std::vector<char> gamma_set;
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  bool pending = false;
+  hipGraphExec_t exec = nullptr;
+  std::vector<hipGraphNode_t> in_nodes, out_nodes;
+  std::vector<const void *> cur_in;
+  std::vector<float *> cur_out;
+};
+
+extern "C" void ncs_hip_pipeline_destroy(ncs_pipeline *pl) {
+  if (!pl) return;
+  if (pl->ctx) (void)hipSetDevice(pl->ctx->device);
+  if (pl->pending) (void)hipEventSynchronize(pl->done);
+  if (pl->exec) (void)hipGraphExecDestroy(pl->exec);
+  DevBuf *bufs[] = {&pl->raw, &pl->data, &pl->gamma, &pl->out, &pl->offset, &pl->gain,
+                    &pl->frames, &pl->queue, &pl->iters, &pl->status, &pl->cost, &pl->u0};
+  for (DevBuf *b : bufs) b->release();
+  if (pl->h_iters) (void)hipHostFree(pl->h_iters);
+  if (pl->h_status) (void)hipHostFree(pl->h_status);
+  if (pl->h_cost) (void)hipHostFree(pl->h_cost);
+  if (pl->done) (void)hipEventDestroy(pl->done);
+  if (pl->stream) (void)hipStreamDestroy(pl->stream);
+  delete pl;
+}
+
+static size_t frame_pixels(const ncs_pipeline_frame &f) { return (size_t)f.height * f.width; }
+
+extern "C" int ncs_hip_pipeline_create(ncs_ctx *ctx, const ncs_pipeline_frame *frames,
+                                       int n_frames, const ncs_params *params,
+                                       int input_format, int keep_tile_results,
+                                       ncs_pipeline **out) {
+  if (!out) return fail(NCS_E_ARG, "out is NULL");
+  *out = nullptr;
+  if (!ctx || !frames || n_frames <= 0) return fail(NCS_E_ARG, "bad arguments");
+  int rc = check_params(params);
+  if (rc) return rc;
+  if (params->start == NCS_START_TILES)
+    return fail(NCS_E_ARG, "a pipeline takes the plain or the learned start");
+  if (params->start == NCS_START_LEARNED && !ctx->has_net)
+    return fail(NCS_E_NOWEIGHTS, "learned start requested but no network weights are set");
+  if (input_format != NCS_INPUT_PHOTONS_F32 && input_format != NCS_INPUT_ADU_U16)
+    return fail(NCS_E_ARG, "unknown input format");
+  if (!ctx->has_mask) return fail(NCS_E_NOMASK, "no OTF mask set");
+  std::unique_ptr<ncs_pipeline, void (*)(ncs_pipeline *)> pl(new ncs_pipeline(),
+                                                             ncs_hip_pipeline_destroy);
+  pl->ctx = ctx;
+  pl->n = n_frames;
+  pl->shape.assign(frames, frames + n_frames);
+  pl->params = *params;
+  pl->input = input_format;
+  pl->elem = input_format == NCS_INPUT_ADU_U16 ? 2 : 4;
+  pl->keep = keep_tile_results != 0;
+  pl->learned = params->start == NCS_START_LEARNED;
+  std::vector<FrameDev> fd(n_frames);
+  pl->off.resize(n_frames);
+  for (int i = 0; i < n_frames; i++) {
+    const ncs_pipeline_frame &f = frames[i];
+    long long nt = ncs_hip_tile_count(f.height, f.width, f.geometry);
+    if (nt < 0) return fail(NCS_E_ARG, "frame " + std::to_string(i) + ": bad size or geometry");
+    if (!std::isfinite(f.alpha)) return fail(NCS_E_ARG, "non-finite alpha");
+    pl->off[i] = pl->pix;
+    pl->pix += (frame_pixels(f) + 63) / 64 * 64;  // 256-byte aligned frames
+    FrameDev &d = fd[i];
+    d.H = f.height;
+    d.W = f.width;
+    d.ds = d.gs = d.os = f.width;
+    d.alpha = f.alpha;
+    d.geom = f.geometry;
+    d.ty = (int)tiles_axis(f.width, f.geometry);
+    d.pad = 0;
+    d.first = pl->total;
+    pl->total += nt;
+    if (f.geometry != NCS_GEOM_REFERENCE) pl->pad = true;
+  }
+  if (pl->total > 0xF0000000LL) return fail(NCS_E_ARG, "too many tiles in one batch");
+  HIPCHK(set_device(ctx));
+  const size_t fb = pl->pix * sizeof(float);
+  HIPCHK(pl->data.ensure(fb));
+  HIPCHK(pl->gamma.ensure(fb));
+  HIPCHK(pl->out.ensure(fb));
+  if (pl->input == NCS_INPUT_ADU_U16) {
+    HIPCHK(pl->raw.ensure(pl->pix * 2));
+    HIPCHK(pl->offset.ensure(fb));
+    HIPCHK(pl->gain.ensure(fb));
+  }
+  HIPCHK(pl->queue.ensure(256));
+  HIPCHK(pl->iters.ensure((size_t)pl->total * 4));
+  HIPCHK(pl->status.ensure((size_t)pl->total * 4));
+  HIPCHK(pl->cost.ensure((size_t)pl->total * 4));
+  if (pl->learned) HIPCHK(pl->u0.ensure((size_t)pl->total * kTilePix * sizeof(float)));
+  if (pl->keep) {
+    HIPCHK(hipHostMalloc((void **)&pl->h_iters, (size_t)pl->total * 4, hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void **)&pl->h_status, (size_t)pl->total * 4, hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void **)&pl->h_cost, (size_t)pl->total * 4, hipHostMallocDefault));
+  }
+  for (int i = 0; i < n_frames; i++) {
+    fd[i].data = pl->data.as<float>() + pl->off[i];
+    fd[i].gamma = pl->gamma.as<float>() + pl->off[i];
+    fd[i].out = pl->out.as<float>() + pl->off[i];
+  }
+  HIPCHK(pl->frames.ensure(fd.size() * sizeof(FrameDev)));
+  HIPCHK(hipMemcpy(pl->frames.p, fd.data(), fd.size() * sizeof(FrameDev), hipMemcpyHostToDevice));
+  HIPCHK(hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking));
+  HIPCHK(hipEventCreateWithFlags(&pl->done, hipEventDisableTiming));
+  pl->gamma_set.assign(n_frames, 0);
+  pl->sched = resolve_schedule(ctx, params, pl->pad);
+  *out = pl.release();
+  return NCS_OK;
+}
+
+static int pipeline_idle(ncs_pipeline *pl) {
+  if (pl->pending) {
+    HIPCHK(hipEventSynchronize(pl->done));
+    pl->pending = false;
+  }
+  return NCS_OK;
+}
+
+extern "C" int ncs_hip_pipeline_set_gamma(ncs_pipeline *pl, int frame, const float *gamma) {
+  if (!pl || !gamma || frame < 0 || frame >= pl->n) return fail(NCS_E_ARG, "bad arguments");
+  HIPCHK(set_device(pl->ctx));
+  int rc = pipeline_idle(pl);
+  if (rc) return rc;
+  HIPCHK(hipMemcpy(pl->gamma.as<float>() + pl->off[frame], gamma,
+                   frame_pixels(pl->shape[frame]) * sizeof(float), hipMemcpyHostToDevice));
+  pl->gamma_set[frame] = 1;
+  return NCS_OK;
+}
+
+extern "C" int ncs_hip_pipeline_set_camera(ncs_pipeline *pl, int frame, const float *offset,
+                                           const float *gain, const float *variance) {
+  if (!pl || !offset || !gain || !variance || frame < 0 || frame >= pl->n)
+    return fail(NCS_E_ARG, "bad arguments");
+  if (pl->input != NCS_INPUT_ADU_U16)
+    return fail(NCS_E_ARG, "camera maps belong to NCS_INPUT_ADU_U16 pipelines");
+  HIPCHK(set_device(pl->ctx));
+  int rc = pipeline_idle(pl);
+  if (rc) return rc;
+  const size_t n = frame_pixels(pl->shape[frame]);
+  const size_t o = pl->off[frame];
+  HIPCHK(hipMemcpy(pl->offset.as<float>() + o, offset, n * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(pl->gain.as<float>() + o, gain, n * 4, hipMemcpyHostToDevice));
+  // the variance goes through the output buffer, which every replay rewrites
+  HIPCHK(hipMemcpy(pl->out.as<float>() + o, variance, n * 4, hipMemcpyHostToDevice));
+  HIPCHK(launch_gamma_from_camera(pl->out.as<float>() + o, pl->gain.as<float>() + o,
+                                  pl->gamma.as<float>() + o, (long long)n, pl->stream));
+  HIPCHK(hipStreamSynchronize(pl->stream));
+  pl->gamma_set[frame] = 1;
+  return NCS_OK;
+}
+
+// Device work of one replay, captured on the pipeline's stream.
+static int capture_body(ncs_pipeline *pl, hipGraph_t *body) {
+  ncs_ctx *ctx = pl->ctx;
+  hipStream_t s = pl->stream;
+  HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+  hipError_t e = hipSuccess;
+  if (pl->input == NCS_INPUT_ADU_U16) {
+    for (int i = 0; i < pl->n && e == hipSuccess; i++) {
+      const size_t o = pl->off[i];
+      e = launch_adu_to_photons(pl->raw.as<uint16_t>() + o, pl->offset.as<float>() + o,
+                                pl->gain.as<float>() + o, pl->data.as<float>() + o,
+                                (long long)frame_pixels(pl->shape[i]), s);
+    }
+  }
+  const float *u0 = nullptr;
+  if (e == hipSuccess && pl->learned) {
+    int grid = (int)std::min<long long>(pl->total, (long long)ctx->cus * 4);
+    e = net_warm(pl->frames.as<FrameDev>(), pl->n, pl->total, pl->u0.as<float>(),
+                 ctx->net_packs.p, ctx->net_params.as<float>(), grid, s);
+    u0 = pl->u0.as<float>();
+  }
+  if (e == hipSuccess) {
+    const SolveArgs a = make_args(ctx, pl->frames.as<FrameDev>(), pl->n, pl->total, &pl->params,
+                                  u0, nullptr, pl->iters.as<int>(), pl->status.as<int>(),
+                                  pl->cost.as<float>(), pl->queue.as<unsigned int>());
+    e = launch_schedule(ctx, pl->sched, pl->pad, pl->total, s, a);
+  }
+  hipGraph_t g = nullptr;
+  hipError_t e2 = hipStreamEndCapture(s, &g);
+  if (e != hipSuccess || e2 != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    HIPCHK(e);
+    HIPCHK(e2);
+  }
+  *body = g;
+  return NCS_OK;
+}
+
+static int build_pipeline_graph(ncs_pipeline *pl, const void *const *inputs,
+                                float *const *outputs) {
+  hipGraph_t body = nullptr, graph = nullptr;
+  int rc = capture_body(pl, &body);
+  if (rc) return rc;
+  const int n = pl->n;
+  std::vector<hipGraphNode_t> in_nodes(n), out_nodes(n), info_nodes;
+  hipGraphNode_t body_node = nullptr;
+  hipError_t e = hipGraphCreate(&graph, 0);
+  char *dev_in = pl->input == NCS_INPUT_ADU_U16 ? pl->raw.as<char>() : pl->data.as<char>();
+  for (int i = 0; i < n && e == hipSuccess; i++)
+    e = hipGraphAddMemcpyNode1D(&in_nodes[i], graph, nullptr, 0, dev_in + pl->off[i] * pl->elem,
+                                inputs[i], frame_pixels(pl->shape[i]) * pl->elem,
+                                hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipGraphAddChildGraphNode(&body_node, graph, in_nodes.data(), (size_t)n, body);
+  for (int i = 0; i < n && e == hipSuccess; i++)
+    e = hipGraphAddMemcpyNode1D(&out_nodes[i], graph, &body_node, 1, outputs[i],
+                                pl->out.as<float>() + pl->off[i],
+                                frame_pixels(pl->shape[i]) * sizeof(float),
+                                hipMemcpyDeviceToHost);
+  if (e == hipSuccess && pl->keep) {
+    const size_t b = (size_t)pl->total * 4;
+    hipGraphNode_t nd;
+    e = hipGraphAddMemcpyNode1D(&nd, graph, &body_node, 1, pl->h_iters, pl->iters.p, b,
+                                hipMemcpyDeviceToHost);
+    if (e == hipSuccess)
+      e = hipGraphAddMemcpyNode1D(&nd, graph, &body_node, 1, pl->h_status, pl->status.p, b,
+                                  hipMemcpyDeviceToHost);
+    if (e == hipSuccess)
+      e = hipGraphAddMemcpyNode1D(&nd, graph, &body_node, 1, pl->h_cost, pl->cost.p, b,
+                                  hipMemcpyDeviceToHost);
+  }
+  hipGraphExec_t exec = nullptr;
+  if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (body) (void)hipGraphDestroy(body);
+  HIPCHK(e);
+  pl->exec = exec;
+  pl->general = pl->ctx->general;
+  pl->in_nodes = in_nodes;
+  pl->out_nodes = out_nodes;
+  pl->cur_in.assign(inputs, inputs + n);
+  pl->cur_out.assign(outputs, outputs + n);
+  return NCS_OK;
+}
+
+extern "C" int ncs_hip_pipeline_launch(ncs_pipeline *pl, const void *const *inputs,
+                                       float *const *outputs, void *stream) {
+  if (!pl) return fail(NCS_E_ARG, "bad arguments");
+  if (!pl->exec && (!inputs || !outputs))
+    return fail(NCS_E_ARG, "the first launch needs inputs and outputs");
+  for (int i = 0; i < pl->n; i++) {
+    if ((inputs && !inputs[i]) || (outputs && !outputs[i]))
+      return fail(NCS_E_ARG, "NULL frame buffer");
+    if (!pl->gamma_set[i])
+      return fail(NCS_E_ARG, "gamma of frame " + std::to_string(i) + " is not set");
+  }
+  ncs_ctx *ctx = pl->ctx;
+  if (!ctx->has_mask) return fail(NCS_E_NOMASK, "no OTF mask set");
+  if (pl->learned && !ctx->has_net) return fail(NCS_E_NOWEIGHTS, "no network weights set");
+  HIPCHK(set_device(ctx));
+  int rc = pipeline_idle(pl);
+  if (rc) return rc;
+  if (!pl->exec) {
+    rc = build_pipeline_graph(pl, inputs, outputs);
+    if (rc) return rc;
+  } else {
+    if (ctx->general != pl->general)
+      return fail(NCS_E_MASK, "the kind of the OTF mask changed since the pipeline was "
+                  "captured; create a new pipeline");
+    char *dev_in = pl->input == NCS_INPUT_ADU_U16 ? pl->raw.as<char>() : pl->data.as<char>();
+    for (int i = 0; i < pl->n; i++) {
+      if (inputs && inputs[i] != pl->cur_in[i]) {
+        HIPCHK(hipGraphExecMemcpyNodeSetParams1D(
+            pl->exec, pl->in_nodes[i], dev_in + pl->off[i] * pl->elem, inputs[i],
+            frame_pixels(pl->shape[i]) * pl->elem, hipMemcpyHostToDevice));
+        pl->cur_in[i] = inputs[i];
+      }
+      if (outputs && outputs[i] != pl->cur_out[i]) {
+        HIPCHK(hipGraphExecMemcpyNodeSetParams1D(
+            pl->exec, pl->out_nodes[i], outputs[i], pl->out.as<float>() + pl->off[i],
+            frame_pixels(pl->shape[i]) * sizeof(float), hipMemcpyDeviceToHost));
+        pl->cur_out[i] = outputs[i];
+      }
+    }
+  }
+  hipStream_t s = stream ? (hipStream_t)stream : pl->stream;
+  HIPCHK(hipGraphLaunch(pl->exec, s));
+  HIPCHK(hipEventRecord(pl->done, s));
+  pl->pending = true;
+  return NCS_OK;
+}
+
+extern "C" int ncs_hip_pipeline_wait(ncs_pipeline *pl) {
+  if (!pl) return fail(NCS_E_ARG, "bad arguments");
+  HIPCHK(set_device(pl->ctx));
+  return pipeline_idle(pl);
+}
+
+extern "C" int ncs_hip_pipeline_tile_results(ncs_pipeline *pl, int *iterations, int *status,
+                                             float *cost) {
+  if (!pl) return fail(NCS_E_ARG, "bad arguments");
+  if (!pl->keep) return fail(NCS_E_ARG, "the pipeline was created without tile results");
+  HIPCHK(set_device(pl->ctx));
+  int rc = pipeline_idle(pl);
+  if (rc) return rc;
+  const size_t b = (size_t)pl->total * 4;
+  if (iterations) std::memcpy(iterations, pl->h_iters, b);
+  if (status) std::memcpy(status, pl->h_status, b);
+  if (cost) std::memcpy(cost, pl->h_cost, b);
+  return NCS_OK;
+}
+
+extern "C" long long ncs_hip_pipeline_tiles(const ncs_pipeline *pl) {
+  return pl ? pl->total : -1;
 }

@@ -12,6 +12,7 @@ constexpr int kTile = 16;          // tile edge (the reference's r_size = 16)
 constexpr int kTilePix = 256;      // pixels per tile
 constexpr int kWave = 64;          // gfx950 wavefront
 constexpr int kWavesPerBlock = 4;  // 256-thread workgroups, one tile per wave
+constexpr int kMaxWavesPerGroup = 4;  // persistent solver: waves per workgroup 1..4
 constexpr float kFitMin = 1.0e-6f; // negative-value guard (ncs.cl:45, ncs.c:169)
 
 // Per-lane OTF tables: 16 floats per lane (see ncs_solver.hip, MaskTables).

@@ -429,18 +429,21 @@ __device__ __forceinline__ void ll_trial(float t, const float (&u)[4], const flo
   wave_sum<2>(red);
 }
 
-// One wave per workgroup; the wave's L-BFGS history lives in LDS as two rings
-// of M vectors (each lane keeps its own 4 values at float4 index
-// slot * 64 + lane, so no lane reads another lane's LDS and no barrier is
-// needed).  PAD enables zero-padded pixels (block geometry).
+// One to kMaxWavesPerGroup waves per workgroup, each solving its own tiles;
+// a wave's L-BFGS history lives in its slice of LDS as two rings of M vectors
+// (each lane keeps its own 4 values at float4 index slot * 64 + lane, so no
+// lane reads another lane's LDS and no barrier is needed).  PAD enables
+// zero-padded pixels (block geometry).
 template <int M, bool GEN, bool PAD>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8)))
+__global__ __launch_bounds__(64 * kMaxWavesPerGroup) __attribute__((amdgpu_waves_per_eu(4, 8)))
 void solve_kernel(SolveArgs A) {
   extern __shared__ float4 lds_hist[];
-  __shared__ MtState mt_lds;              // line-search state of this wave
-  float4 *const Sb = lds_hist;            // s vectors, ring of M
-  float4 *const Yb = lds_hist + M * 64;   // y vectors, ring of M
-  const int lane = threadIdx.x;
+  __shared__ MtState mt_state[kMaxWavesPerGroup];  // line-search state per wave
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform
+  MtState &mt_lds = mt_state[wave];
+  float4 *const Sb = lds_hist + wave * (2 * M * 64);  // s vectors, ring of M
+  float4 *const Yb = Sb + M * 64;                     // y vectors, ring of M
+  const int lane = threadIdx.x & 63;
   const int col = lane & 15, grp = lane >> 4;
 
   MaskTables T;
@@ -849,26 +852,32 @@ static SolveKernelFn solve_kernel_for(int m, bool general, bool pad) {
   }
 }
 
-static size_t solve_lds_bytes(int m) { return (size_t)m * 2 * 64 * sizeof(float4); }
-
-// Waves of this instantiation one CU holds at once (0 if unsupported).
-int solve_waves_per_cu(int m, bool general, bool pad) {
-  SolveKernelFn fn = solve_kernel_for(m, general, pad);
-  if (!fn) return 0;
-  int blocks = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-          &blocks, reinterpret_cast<const void *>(fn), kWave, solve_lds_bytes(m)) !=
-      hipSuccess)
-    return 0;
-  return blocks;
+static size_t solve_lds_bytes(int m, int wpg) {
+  return (size_t)wpg * m * 2 * 64 * sizeof(float4);
 }
 
-hipError_t launch_solve(int m, bool general, bool pad, unsigned int waves, hipStream_t s,
-                        const SolveArgs &a) {
+// Waves of this instantiation one CU holds at once with workgroups of `wpg`
+// waves (0 if unsupported).
+int solve_waves_per_cu(int m, bool general, bool pad, int wpg) {
   SolveKernelFn fn = solve_kernel_for(m, general, pad);
-  if (!fn) return hipErrorInvalidValue;
+  if (!fn || wpg < 1 || wpg > kMaxWavesPerGroup) return 0;
+  int blocks = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &blocks, reinterpret_cast<const void *>(fn), kWave * wpg, solve_lds_bytes(m, wpg)) !=
+      hipSuccess)
+    return 0;
+  return blocks * wpg;
+}
+
+// Resets the queue and launches `waves` persistent waves in workgroups of
+// `wpg` waves (waves is rounded up to a multiple of wpg).
+hipError_t launch_solve(int m, bool general, bool pad, unsigned int waves, int wpg,
+                        hipStream_t s, const SolveArgs &a) {
+  SolveKernelFn fn = solve_kernel_for(m, general, pad);
+  if (!fn || wpg < 1 || wpg > kMaxWavesPerGroup || waves == 0) return hipErrorInvalidValue;
+  const unsigned int groups = (waves + wpg - 1) / wpg;
   hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(64), 0, s, a.queue);
-  hipLaunchKernelGGL(fn, dim3(waves), dim3(kWave), solve_lds_bytes(m), s, a);
+  hipLaunchKernelGGL(fn, dim3(groups), dim3(kWave * wpg), solve_lds_bytes(m, wpg), s, a);
   return hipGetLastError();
 }
 
