@@ -133,10 +133,17 @@ def run_gpu(frames_np, args, weights):
     # liblbfgs's scalar initial inverse Hessian s.y / y.y (ncs.cl:1123)
     scalar = _lib.default_params(history=HISTORY, max_iterations=MAX_IT, precond=0)
     timed(scalar, "plain_scalar_h0", max(1, min(args.steps, 3)), 1)
+    # the autotuner also picks the history depth (history=0 below)
+    tune_h = ctx.autotune(desc[:1], _lib.default_params(max_iterations=MAX_IT),
+                          tune_history=True)
+    print("[bench] autotune with depth: %s" % tune_h, file=sys.stderr, flush=True)
+    results["tune_history"] = tune_h
+    timed(_lib.default_params(history=0, max_iterations=MAX_IT), "tuned", args.steps,
+          args.warmup)
     if weights is not None:
         from ncs_amd import warmstart
         warmstart.set_context_weights(weights)
-        learned = _lib.default_params(history=HISTORY, max_iterations=MAX_IT,
+        learned = _lib.default_params(history=0, max_iterations=MAX_IT,
                                       start=_lib.START_LEARNED)
         timed(learned, "learned", args.steps, args.warmup)
         # the net alone, and the fallback decision on the first frame
@@ -618,15 +625,39 @@ def main():
     plain = res["plain"]
     pix = len(frames_np) * H * W
     mpix = pix / plain["wall_s"] / 1e6
-    kms = plain["kernel_ms"]
-    it = plain["iters"].astype(numpy.int64)
-    ev = it + 1  # one accepted trial per iteration plus the start (lower bound)
-    flops = float(numpy.sum(flops_per_tile(it, ev, HISTORY, precond=1)))
-    achieved = flops / (kms * 1e-3) / 1e12
     alg_bytes = pix * 12.0  # data + gamma in, image out (float32)
+
+    def roofline_of(run, m):
+        kms = run["kernel_ms"]
+        it = run["iters"].astype(numpy.int64)
+        ev = it + 1  # one accepted trial per iteration plus the start (lower bound)
+        flops = float(numpy.sum(flops_per_tile(it, ev, m, precond=1)))
+        achieved = flops / (kms * 1e-3) / 1e12
+        return {"bound": "mfma", "achieved": round(achieved, 3),
+                "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                "kernel": "solve_kernel<%d,false,false>" % m, "kernel_ms": round(kms, 3),
+                "flop_per_launch": flops,
+                "hbm_gbs_algorithmic": round(alg_bytes / (kms * 1e-3) / 1e9, 2)}
 
     parity, _ = parity_sample(frames_np, plain, ntile_frame, 1)
     value, start_used = mpix, "plain"
+    candidates = {"plain": (mpix, True, plain["wall_s"])}
+    tuned_info = None
+    if "tuned" in res:
+        T = res["tuned"]
+        tm = pix / T["wall_s"] / 1e6
+        rel = numpy.abs(T["cost"].astype(numpy.float64) - plain["cost"]) / numpy.abs(plain["cost"])
+        tuned_info = {"value": round(tm, 2), "unit": "Mpix/s",
+                      "ms_per_step": round(T["wall_s"] * 1e3, 3),
+                      "kernel_ms": round(T["kernel_ms"], 3),
+                      "history": res["tune_history"]["history"],
+                      "waves_per_cu": res["tune_history"]["waves_per_cu"],
+                      "waves_per_group": res["tune_history"]["waves_per_group"],
+                      "iterations_mean": round(float(T["iters"].mean()), 3),
+                      "cost_vs_plain_max_rel": float(rel.max()),
+                      "matched_1e5": bool(rel.max() <= 1e-5)}
+        candidates["tuned"] = (tm, tuned_info["matched_1e5"], T["wall_s"])
     learned_info = None
     if "learned" in res:
         L = res["learned"]
@@ -645,10 +676,16 @@ def main():
             "cost_vs_plain_p99_rel": float(numpy.percentile(rel, 99)),
             "matched_1e5": matched,
             "speedup_vs_plain": round(lm / mpix, 3),
+            "speedup_vs_plain_scalar_h0": (round(lm * res["plain_scalar_h0"]["wall_s"] / pix * 1e6, 3)
+                                           if "plain_scalar_h0" in res else None),
             "weights": weights_note,
             "status": {str(k): int(v) for k, v in zip(*numpy.unique(L["status"], return_counts=True))}}
-        if matched:
-            value, start_used = lm, "learned"
+        candidates["learned"] = (lm, matched, L["wall_s"])
+    # headline: the fastest path whose per-tile costs match the plain run's
+    # (m = 5, spectral H0) within 1e-5
+    for name, (v, ok, _) in candidates.items():
+        if ok and v > value:
+            value, start_used = v, name
     line = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -656,31 +693,29 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round((res["learned"]["wall_s"] if start_used == "learned" else plain["wall_s"]) * 1e3, 3),
+        "ms_per_step": round(candidates[start_used][2] * 1e3, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (WLC filament scenes, sCMOS camera maps; seeds 1-%d)" % len(frames_np),
-        "config": {"workload": "plain NCS batched: %d frames of %dx%d, alpha %.1f, m=%d, "
-                               "<=%d iterations, eps 1e-5, reference tiling"
-                               % (len(frames_np), H, W, ALPHA, HISTORY, MAX_IT),
+        "config": {"workload": "plain NCS batched: %d frames of %dx%d, alpha %.1f, m=%d "
+                               "reference run, <=%d iterations, eps 1e-5, reference tiling; "
+                               "headline path: %s"
+                               % (len(frames_np), H, W, ALPHA, HISTORY, MAX_IT, start_used),
                    "frames": len(frames_np), "height": H, "width": W, "alpha": ALPHA,
                    "history": HISTORY, "max_iterations": MAX_IT, "epsilon": 1e-5,
-                   "tiles": int(total), "start": start_used},
+                   "tiles": int(total), "path": start_used},
         "plain": {"value": round(mpix, 2), "unit": "Mpix/s",
                   "ms_per_step": round(plain["wall_s"] * 1e3, 3),
                   "kernel_ms": round(plain["kernel_ms"], 3),
                   "initial_hessian": "spectral (precond=1)"},
         "plain_scalar_h0": _scalar_summary(res, plain, pix),
+        "tuned": tuned_info,
         "autotune": res.get("tune"),
         "learned_warm": learned_info,
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 3),
-                     "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
-                     "kernel": "solve_kernel<5,false>", "kernel_ms": round(kms, 3),
-                     "flop_per_launch": flops,
-                     "hbm_gbs_algorithmic": round(alg_bytes / (kms * 1e-3) / 1e9, 2)},
+        "roofline": (roofline_of(res["tuned"], res["tune_history"]["history"])
+                     if start_used == "tuned" else roofline_of(plain, HISTORY)),
         "tiles_per_s": round(total / plain["wall_s"], 1),
         "tiles_per_s_nominal_16x16": round(len(frames_np) * (H // 16) * (W // 16) / plain["wall_s"], 1),
         "iterations": {"mean": round(float(it.mean()), 3), "p50": float(numpy.median(it)),

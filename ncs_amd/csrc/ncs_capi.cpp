@@ -804,7 +804,7 @@ extern "C" int ncs_hip_autotune(ncs_ctx *ctx, const ncs_frame *frames, int n_fra
   float ms_best = ms_def;
   int tried = 1, best_occ = def.waves_per_cu;
   std::vector<int> depths;
-  if (tune_history) depths = {3, 4, 5, 6, 8};
+  if (tune_history) depths = {1, 2, 3, 5, 8};
   else depths = {def.history};
   const int groups[] = {1, 2, 4};
   for (int m : depths) {
@@ -873,7 +873,6 @@ struct ncs_pipeline {
   hipEvent_t done = nullptr;
   bool pending = false;
   hipGraphExec_t exec = nullptr;
-  std::vector<hipGraphNode_t> in_nodes, out_nodes;
   std::vector<const void *> cur_in;
   std::vector<float *> cur_out;
 };
@@ -1020,12 +1019,20 @@ extern "C" int ncs_hip_pipeline_set_camera(ncs_pipeline *pl, int frame, const fl
   return NCS_OK;
 }
 
-// Device work of one replay, captured on the pipeline's stream.
-static int capture_body(ncs_pipeline *pl, hipGraph_t *body) {
+// One replay captured on the pipeline's stream as a single chain: upload of
+// every frame, camera conversion, learned start, queue reset and solve,
+// download of every frame and of the per-tile results.  Stream order becomes
+// the graph's edges.
+static int capture_replay(ncs_pipeline *pl, const void *const *inputs, float *const *outputs,
+                          hipGraph_t *out) {
   ncs_ctx *ctx = pl->ctx;
   hipStream_t s = pl->stream;
   HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
   hipError_t e = hipSuccess;
+  char *dev_in = pl->input == NCS_INPUT_ADU_U16 ? pl->raw.as<char>() : pl->data.as<char>();
+  for (int i = 0; i < pl->n && e == hipSuccess; i++)
+    e = hipMemcpyAsync(dev_in + pl->off[i] * pl->elem, inputs[i],
+                       frame_pixels(pl->shape[i]) * pl->elem, hipMemcpyHostToDevice, s);
   if (pl->input == NCS_INPUT_ADU_U16) {
     for (int i = 0; i < pl->n && e == hipSuccess; i++) {
       const size_t o = pl->off[i];
@@ -1047,6 +1054,15 @@ static int capture_body(ncs_pipeline *pl, hipGraph_t *body) {
                                   pl->cost.as<float>(), pl->queue.as<unsigned int>());
     e = launch_schedule(ctx, pl->sched, pl->pad, pl->total, s, a);
   }
+  for (int i = 0; i < pl->n && e == hipSuccess; i++)
+    e = hipMemcpyAsync(outputs[i], pl->out.as<float>() + pl->off[i],
+                       frame_pixels(pl->shape[i]) * sizeof(float), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess && pl->keep) {
+    const size_t b = (size_t)pl->total * 4;
+    e = hipMemcpyAsync(pl->h_iters, pl->iters.p, b, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(pl->h_status, pl->status.p, b, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(pl->h_cost, pl->cost.p, b, hipMemcpyDeviceToHost, s);
+  }
   hipGraph_t g = nullptr;
   hipError_t e2 = hipStreamEndCapture(s, &g);
   if (e != hipSuccess || e2 != hipSuccess) {
@@ -1054,54 +1070,37 @@ static int capture_body(ncs_pipeline *pl, hipGraph_t *body) {
     HIPCHK(e);
     HIPCHK(e2);
   }
-  *body = g;
+  *out = g;
   return NCS_OK;
 }
 
+// Captures the replay for these host buffers and instantiates it, or updates
+// the existing executable graph in place (same topology, new pointers).
 static int build_pipeline_graph(ncs_pipeline *pl, const void *const *inputs,
                                 float *const *outputs) {
-  hipGraph_t body = nullptr, graph = nullptr;
-  int rc = capture_body(pl, &body);
+  hipGraph_t graph = nullptr;
+  int rc = capture_replay(pl, inputs, outputs, &graph);
   if (rc) return rc;
-  const int n = pl->n;
-  std::vector<hipGraphNode_t> in_nodes(n), out_nodes(n), info_nodes;
-  hipGraphNode_t body_node = nullptr;
-  hipError_t e = hipGraphCreate(&graph, 0);
-  char *dev_in = pl->input == NCS_INPUT_ADU_U16 ? pl->raw.as<char>() : pl->data.as<char>();
-  for (int i = 0; i < n && e == hipSuccess; i++)
-    e = hipGraphAddMemcpyNode1D(&in_nodes[i], graph, nullptr, 0, dev_in + pl->off[i] * pl->elem,
-                                inputs[i], frame_pixels(pl->shape[i]) * pl->elem,
-                                hipMemcpyHostToDevice);
-  if (e == hipSuccess)
-    e = hipGraphAddChildGraphNode(&body_node, graph, in_nodes.data(), (size_t)n, body);
-  for (int i = 0; i < n && e == hipSuccess; i++)
-    e = hipGraphAddMemcpyNode1D(&out_nodes[i], graph, &body_node, 1, outputs[i],
-                                pl->out.as<float>() + pl->off[i],
-                                frame_pixels(pl->shape[i]) * sizeof(float),
-                                hipMemcpyDeviceToHost);
-  if (e == hipSuccess && pl->keep) {
-    const size_t b = (size_t)pl->total * 4;
-    hipGraphNode_t nd;
-    e = hipGraphAddMemcpyNode1D(&nd, graph, &body_node, 1, pl->h_iters, pl->iters.p, b,
-                                hipMemcpyDeviceToHost);
-    if (e == hipSuccess)
-      e = hipGraphAddMemcpyNode1D(&nd, graph, &body_node, 1, pl->h_status, pl->status.p, b,
-                                  hipMemcpyDeviceToHost);
-    if (e == hipSuccess)
-      e = hipGraphAddMemcpyNode1D(&nd, graph, &body_node, 1, pl->h_cost, pl->cost.p, b,
-                                  hipMemcpyDeviceToHost);
+  hipError_t e = hipSuccess;
+  bool updated = false;
+  if (pl->exec) {
+    hipGraphNode_t err_node = nullptr;
+    hipGraphExecUpdateResult res;
+    updated = hipGraphExecUpdate(pl->exec, graph, &err_node, &res) == hipSuccess &&
+              res == hipGraphExecUpdateSuccess;
+    if (!updated) {
+      (void)hipGetLastError();
+      (void)hipGraphExecDestroy(pl->exec);
+      pl->exec = nullptr;
+    }
   }
-  hipGraphExec_t exec = nullptr;
-  if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-  if (graph) (void)hipGraphDestroy(graph);
-  if (body) (void)hipGraphDestroy(body);
+  if (!updated) e = hipGraphInstantiate(&pl->exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) pl->exec = nullptr;
   HIPCHK(e);
-  pl->exec = exec;
   pl->general = pl->ctx->general;
-  pl->in_nodes = in_nodes;
-  pl->out_nodes = out_nodes;
-  pl->cur_in.assign(inputs, inputs + n);
-  pl->cur_out.assign(outputs, outputs + n);
+  pl->cur_in.assign(inputs, inputs + pl->n);
+  pl->cur_out.assign(outputs, outputs + pl->n);
   return NCS_OK;
 }
 
@@ -1122,28 +1121,19 @@ extern "C" int ncs_hip_pipeline_launch(ncs_pipeline *pl, const void *const *inpu
   HIPCHK(set_device(ctx));
   int rc = pipeline_idle(pl);
   if (rc) return rc;
-  if (!pl->exec) {
-    rc = build_pipeline_graph(pl, inputs, outputs);
+  if (pl->exec && ctx->general != pl->general)
+    return fail(NCS_E_MASK, "the kind of the OTF mask changed since the pipeline was "
+                "captured; create a new pipeline");
+  bool same = pl->exec != nullptr;
+  for (int i = 0; i < pl->n && same; i++)
+    same = (!inputs || inputs[i] == pl->cur_in[i]) && (!outputs || outputs[i] == pl->cur_out[i]);
+  if (!same) {
+    std::vector<const void *> in(pl->cur_in);
+    std::vector<float *> outv(pl->cur_out);
+    if (inputs) in.assign(inputs, inputs + pl->n);
+    if (outputs) outv.assign(outputs, outputs + pl->n);
+    rc = build_pipeline_graph(pl, in.data(), outv.data());
     if (rc) return rc;
-  } else {
-    if (ctx->general != pl->general)
-      return fail(NCS_E_MASK, "the kind of the OTF mask changed since the pipeline was "
-                  "captured; create a new pipeline");
-    char *dev_in = pl->input == NCS_INPUT_ADU_U16 ? pl->raw.as<char>() : pl->data.as<char>();
-    for (int i = 0; i < pl->n; i++) {
-      if (inputs && inputs[i] != pl->cur_in[i]) {
-        HIPCHK(hipGraphExecMemcpyNodeSetParams1D(
-            pl->exec, pl->in_nodes[i], dev_in + pl->off[i] * pl->elem, inputs[i],
-            frame_pixels(pl->shape[i]) * pl->elem, hipMemcpyHostToDevice));
-        pl->cur_in[i] = inputs[i];
-      }
-      if (outputs && outputs[i] != pl->cur_out[i]) {
-        HIPCHK(hipGraphExecMemcpyNodeSetParams1D(
-            pl->exec, pl->out_nodes[i], outputs[i], pl->out.as<float>() + pl->off[i],
-            frame_pixels(pl->shape[i]) * sizeof(float), hipMemcpyDeviceToHost));
-        pl->cur_out[i] = outputs[i];
-      }
-    }
   }
   hipStream_t s = stream ? (hipStream_t)stream : pl->stream;
   HIPCHK(hipGraphLaunch(pl->exec, s));

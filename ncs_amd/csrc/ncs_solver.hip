@@ -158,7 +158,7 @@ __device__ __forceinline__ void apply_h0(float (&x)[4], const MaskTables &T, flo
                                          float alpha) {
   float w[4];
 #pragma unroll
-  for (int r = 0; r < 4; r++) w[r] = 1.0f / fmaf(alpha, T.qe[r], kappa);
+  for (int r = 0; r < 4; r++) w[r] = rcpf(fmaf(alpha, T.qe[r], kappa));
   float tmp[4] = {x[0], x[1], x[2], x[3]};
   rdft_sandwich<false>(tmp, T, w, x);
 }
@@ -374,18 +374,33 @@ __device__ __forceinline__ TileGeom tile_geom(const FrameDev &F, long long local
   return G;
 }
 
-// Per-lane likelihood value and gradient at u (exact logs).
-__device__ __forceinline__ float ll_lane(const float (&u)[4], const float (&gm)[4],
-                                         const float (&c)[4], const float (&lin)[4],
-                                         float (&gll)[4]) {
+// Per-lane likelihood value at u: log2 on the transcendental unit times
+// ln 2 (a few ulp per term; the cost is a sum of terms of one sign-mixed
+// magnitude, so its relative error stays near 1e-7).
+__device__ __forceinline__ float ll_value_lane(const float (&u)[4], const float (&gm)[4],
+                                               const float (&c)[4], const float (&lin)[4]) {
   float s = 0.0f;
 #pragma unroll
   for (int r = 0; r < 4; r++) {
-    float v = fmaxf(u[r] + gm[r], kFitMin);
-    s += lin[r] * u[r] - c[r] * logf(v);
-    gll[r] = lin[r] - c[r] / v;
+    const float v = fmaxf(u[r] + gm[r], kFitMin);
+    s += lin[r] * u[r] - c[r] * __logf(v);
   }
   return s;
+}
+
+// Per-lane likelihood gradient lin - c / max(u + gamma, FITMIN) (ncs.c:158-175).
+__device__ __forceinline__ void ll_grad_lane(const float (&u)[4], const float (&gm)[4],
+                                             const float (&c)[4], const float (&lin)[4],
+                                             float (&gll)[4]) {
+#pragma unroll
+  for (int r = 0; r < 4; r++) gll[r] = lin[r] - c[r] * rcpf(fmaxf(u[r] + gm[r], kFitMin));
+}
+
+__device__ __forceinline__ float ll_lane(const float (&u)[4], const float (&gm)[4],
+                                         const float (&c)[4], const float (&lin)[4],
+                                         float (&gll)[4]) {
+  ll_grad_lane(u, gm, c, lin, gll);
+  return ll_value_lane(u, gm, c, lin);
 }
 
 // One trial of the line search at step t: the likelihood part of
@@ -501,12 +516,14 @@ void solve_kernel(SolveArgs A) {
     // starting point, cost and gradient
     float gnc[4], gll[4], g[4];
     apply_k<GEN>(u, T, gnc);
-    float f;
+    float f = 0.0f;  // f(u) - f(u_start): only differences drive the search
     {
       float red[2];
-      red[0] = ll_lane(u, gm, c, lin, gll);
-      red[1] = u[0] * gnc[0] + u[1] * gnc[1] + (u[2] * gnc[2] + u[3] * gnc[3]);
-      if (A.u0 != nullptr) {
+      if (A.u0 == nullptr) {
+        ll_grad_lane(u, gm, c, lin, gll);
+      } else {
+        red[0] = ll_lane(u, gm, c, lin, gll);
+        red[1] = u[0] * gnc[0] + u[1] * gnc[1] + (u[2] * gnc[2] + u[3] * gnc[3]);
         // warm start: keep it unless the plain start scores a lower cost
         float w[4], wg[4], wll[4];
 #pragma unroll
@@ -523,13 +540,7 @@ void solve_kernel(SolveArgs A) {
         if (!A.fallback || f_warm < f_plain || !(f_plain == f_plain)) {
 #pragma unroll
           for (int r = 0; r < 4; r++) { u[r] = w[r]; gnc[r] = wg[r]; gll[r] = wll[r]; }
-          f = f_warm;
-        } else {
-          f = f_plain;
         }
-      } else {
-        wave_sum<2>(red);
-        f = red[0] + 0.5f * alpha * red[1];
       }
     }
 #pragma unroll
@@ -743,11 +754,11 @@ void solve_kernel(SolveArgs A) {
       step = 1.0f;
     }
 
-    // exact final cost at the returned point
+    // final cost at the returned point, NC from an exact K u
     if (!gnc_exact) apply_k<GEN>(u, T, gnc);
     {
       float red[2];
-      red[0] = ll_lane(u, gm, c, lin, gll);
+      red[0] = ll_value_lane(u, gm, c, lin);
       red[1] = u[0] * gnc[0] + u[1] * gnc[1] + (u[2] * gnc[2] + u[3] * gnc[3]);
       wave_sum<2>(red);
       f = red[0] + 0.5f * alpha * red[1];

@@ -207,7 +207,7 @@ def test_autotune_picks_a_schedule_that_is_used(otf):
     _say("autotune", r)
     assert r["waves_per_group"] in (1, 2, 4)
     assert 1 <= r["waves_per_cu"] <= r["lds_waves_per_cu"]
-    assert r["history"] in (3, 4, 5, 6, 8) and r["candidates"] >= 10
+    assert r["history"] in (1, 2, 3, 5, 8) and r["candidates"] >= 10
     assert 0 < r["ms"] <= r["ms_default"]
     # nothing written to the sample's output
     assert float(keep[2].abs().max()) == 0.0
