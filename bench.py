@@ -626,6 +626,7 @@ def main():
     pix = len(frames_np) * H * W
     mpix = pix / plain["wall_s"] / 1e6
     alg_bytes = pix * 12.0  # data + gamma in, image out (float32)
+    it = plain["iters"].astype(numpy.int64)
 
     def roofline_of(run, m):
         kms = run["kernel_ms"]

@@ -150,7 +150,7 @@ extern "C" int ncs_hip_create(int device, ncs_ctx **out) {
   std::snprintf(c->arch, sizeof(c->arch), "%s", prop.gcnArchName);
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   HIPCHK(hipEventCreateWithFlags(&c->frames_event, hipEventDisableTiming));
-  HIPCHK(c->queue.ensure(256));
+  HIPCHK(c->queue.ensure(kQueueBytes));
   *out = c.release();
   return NCS_OK;
 }
@@ -767,7 +767,7 @@ extern "C" int ncs_hip_autotune(ncs_ctx *ctx, const ncs_frame *frames, int n_fra
     if (e1) (void)hipEventDestroy(e1);
   };
   hipError_t e = dfr.ensure(fd.size() * sizeof(FrameDev));
-  if (e == hipSuccess) e = q.ensure(256);
+  if (e == hipSuccess) e = q.ensure(kQueueBytes);
   if (e == hipSuccess)
     e = hipMemcpy(dfr.p, fd.data(), fd.size() * sizeof(FrameDev), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipEventCreate(&e0);
@@ -953,7 +953,7 @@ extern "C" int ncs_hip_pipeline_create(ncs_ctx *ctx, const ncs_pipeline_frame *f
     HIPCHK(pl->offset.ensure(fb));
     HIPCHK(pl->gain.ensure(fb));
   }
-  HIPCHK(pl->queue.ensure(256));
+  HIPCHK(pl->queue.ensure(kQueueBytes));
   HIPCHK(pl->iters.ensure((size_t)pl->total * 4));
   HIPCHK(pl->status.ensure((size_t)pl->total * 4));
   HIPCHK(pl->cost.ensure((size_t)pl->total * 4));

@@ -13,6 +13,16 @@ constexpr int kTilePix = 256;      // pixels per tile
 constexpr int kWave = 64;          // gfx950 wavefront
 constexpr int kWavesPerBlock = 4;  // 256-thread workgroups, one tile per wave
 constexpr int kMaxWavesPerGroup = 4;  // persistent solver: waves per workgroup 1..4
+
+// Work queue of the persistent solver: the tiles are split into kQueues
+// contiguous ranges, one per XCD (workgroups are dispatched round-robin over
+// the 8 XCDs), each with its own counter on its own 128-byte line.  A wave
+// takes chunks of kChunk tiles from its home range, one tile at a time near
+// the end of a range, then steals from the other ranges.
+constexpr int kQueues = 8;
+constexpr int kQueueStride = 32;   // unsigned ints between counters
+constexpr int kChunk = 8;
+constexpr size_t kQueueBytes = kQueues * kQueueStride * sizeof(unsigned int);
 constexpr float kFitMin = 1.0e-6f; // negative-value guard (ncs.cl:45, ncs.c:169)
 
 // Per-lane OTF tables: 16 floats per lane (see ncs_solver.hip, MaskTables).
@@ -37,7 +47,7 @@ struct SolveArgs {
   const FrameDev *frames;
   int n_frames;
   long long total_tiles;
-  unsigned int *queue;         // device work-queue counter
+  unsigned int *queue;         // kQueues work-queue counters (kQueueBytes)
   const float *tables;         // [64][16] per-lane OTF/DFT tables
   const float *u0;             // [tiles][256] warm start or nullptr
   float *u_tiles;              // [tiles][256] solved tiles or nullptr

@@ -474,11 +474,39 @@ void solve_kernel(SolveArgs A) {
     }
   }
 
+  // work queue: home range of this workgroup's XCD first, then the others
+  const int home = blockIdx.x % kQueues;
+  const long long waves_per_queue =
+      ((long long)gridDim.x * (blockDim.x >> 6) + kQueues - 1) / kQueues;
+  const long long tail = 4LL * kChunk * waves_per_queue;
+  int qv = 0;                   // ranges found exhausted
+  long long next = 0, end = 0;  // this wave's current chunk [next, end)
   for (;;) {
-    unsigned int tq = 0;
-    if (lane == 0) tq = atomicAdd(A.queue, 1u);
-    const long long t = (long long)(unsigned int)__builtin_amdgcn_readfirstlane((int)tq);
-    if (t >= A.total_tiles) break;
+    if (next >= end) {
+      while (qv < kQueues) {
+        const int q = (home + qv) % kQueues;
+        const long long lo = A.total_tiles * q / kQueues;
+        const long long hi = A.total_tiles * (q + 1) / kQueues;
+        unsigned int *ctr = A.queue + q * kQueueStride;
+        unsigned int got = 0, want = 1;
+        if (lane == 0) {
+          const unsigned int seen =
+              __hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          want = (hi - lo - (long long)seen > tail) ? (unsigned int)kChunk : 1u;
+          got = atomicAdd(ctr, want);
+        }
+        got = (unsigned int)__builtin_amdgcn_readfirstlane((int)got);
+        want = (unsigned int)__builtin_amdgcn_readfirstlane((int)want);
+        if (lo + (long long)got < hi) {
+          next = lo + got;
+          end = min(hi, next + (long long)want);
+          break;
+        }
+        qv++;
+      }
+      if (next >= end) break;
+    }
+    const long long t = next++;
 
     // frame of tile t (uniform binary search over the frame table)
     int lo = 0, hi = A.n_frames - 1;
@@ -833,10 +861,10 @@ hipError_t launch_eval(bool general, const float *data, const float *gamma, cons
   return hipGetLastError();
 }
 
-// Resets the work-queue counter (a kernel, not a memset, so that captured
+// Resets the work-queue counters (a kernel, not a memset, so that captured
 // graphs replay it correctly).
 __global__ void reset_queue_kernel(unsigned int *q) {
-  if (threadIdx.x == 0) *q = 0u;
+  if (threadIdx.x < kQueues) q[threadIdx.x * kQueueStride] = 0u;
 }
 
 // Instantiations: history 1..8, with and without the odd-mask coupling and
