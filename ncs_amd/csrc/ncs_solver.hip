@@ -346,13 +346,16 @@ __device__ __forceinline__ int mt_post(MtState &S, float &stp, float f, float dg
 // The persistent solver kernel
 // ---------------------------------------------------------------------------
 
+// The frame table seen through the constant address space (scalar loads).
+typedef __attribute__((address_space(4))) const FrameDev FrameConst;
+
 struct TileGeom {
   int bx, by;           // window origin in the frame (may be -1)
   int r_lo, r_hi;       // owned local rows [r_lo, r_hi)
   int c_lo, c_hi;       // owned local cols
 };
 
-__device__ __forceinline__ TileGeom tile_geom(const FrameDev &F, long long local) {
+__device__ __forceinline__ TileGeom tile_geom(const FrameConst &F, long long local) {
   TileGeom G;
   int a = (int)(local / F.ty), b = (int)(local % F.ty);
   if (F.geom == NCS_GEOM_REFERENCE) {
@@ -474,14 +477,16 @@ void solve_kernel(SolveArgs A) {
     }
   }
 
-  // work queue: home range of this workgroup's XCD first, then the others
+  // Work queue: chunks of the home range of this workgroup's XCD first, then
+  // of the other ranges.  next_tile() gives the wave's next tile (uniform),
+  // or -1 when every range is exhausted.
   const int home = blockIdx.x % kQueues;
   const long long waves_per_queue =
       ((long long)gridDim.x * (blockDim.x >> 6) + kQueues - 1) / kQueues;
   const long long tail = 4LL * kChunk * waves_per_queue;
   int qv = 0;                   // ranges found exhausted
   long long next = 0, end = 0;  // this wave's current chunk [next, end)
-  for (;;) {
+  auto next_tile = [&]() -> long long {
     if (next >= end) {
       while (qv < kQueues) {
         const int q = (home + qv) % kQueues;
@@ -504,41 +509,72 @@ void solve_kernel(SolveArgs A) {
         }
         qv++;
       }
-      if (next >= end) break;
+      if (next >= end) return -1;
     }
-    const long long t = next++;
+    return next++;
+  };
 
-    // frame of tile t (uniform binary search over the frame table)
-    int lo = 0, hi = A.n_frames - 1;
-    while (lo < hi) {
-      int mid = (lo + hi + 1) >> 1;
-      if (A.frames[mid].first <= t) lo = mid; else hi = mid - 1;
+  // The frame table is read through the constant address space (scalar
+  // loads into SGPRs; the kernel never writes it).  Tiles of a chunk are
+  // consecutive, so the previous frame is tried before a binary search.
+  const FrameConst *frames = (const FrameConst *)(A.frames);
+  int fi = 0;
+  auto frame_of = [&](long long t) -> int {
+    if (!(frames[fi].first <= t && (fi + 1 >= A.n_frames || frames[fi + 1].first > t))) {
+      int lo = 0, hi = A.n_frames - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (frames[mid].first <= t) lo = mid; else hi = mid - 1;
+      }
+      fi = lo;
     }
-    const FrameDev F = A.frames[lo];
-    const TileGeom G = tile_geom(F, t - F.first);
-    const float alpha = F.alpha;
+    return fi;
+  };
 
-    // gather (edge clamp for the reference geometry, zero pad for blocks)
-    float gm[4], c[4], lin[4], u[4];
+  // Raw window of a tile (edge clamp for the reference geometry, zero pad
+  // for blocks) and, for warm starts, its start tile.
+  auto load_tile = [&](const FrameConst &F, const TileGeom &G, long long t, float (&dd)[4],
+                       float (&gg)[4], float (&w0)[4]) {
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       const int row = 4 * grp + r;
-      float dd;
       if (!PAD || F.geom == NCS_GEOM_REFERENCE) {
         const int gr = min(max(G.bx + row, 0), F.H - 1);
         const int gc = min(max(G.by + col, 0), F.W - 1);
-        dd = F.data[(long long)gr * F.ds + gc];
-        gm[r] = F.gamma[(long long)gr * F.gs + gc];
-        lin[r] = 1.0f;
+        dd[r] = F.data[(long long)gr * F.ds + gc];
+        gg[r] = F.gamma[(long long)gr * F.gs + gc];
       } else {
         const bool ok = (row < G.r_hi) && (col < G.c_hi);
         const int gr = ok ? G.bx + row : 0, gc = ok ? G.by + col : 0;
-        dd = ok ? F.data[(long long)gr * F.ds + gc] : 0.0f;
-        gm[r] = ok ? F.gamma[(long long)gr * F.gs + gc] : 0.0f;
-        lin[r] = ok ? 1.0f : 0.0f;
+        dd[r] = ok ? F.data[(long long)gr * F.ds + gc] : 0.0f;
+        gg[r] = ok ? F.gamma[(long long)gr * F.gs + gc] : 0.0f;
       }
-      c[r] = lin[r] * (dd + gm[r]);
-      u[r] = dd;
+    }
+    if (A.u0 != nullptr) {
+      const float *w = A.u0 + t * kTilePix + (4 * grp) * kTile + col;
+#pragma unroll
+      for (int r = 0; r < 4; r++) w0[r] = w[r * kTile];
+    }
+  };
+
+  long long t = next_tile();
+  if (t < 0) return;
+  int fc = frame_of(t);  // frame of the current tile
+  TileGeom G = tile_geom(frames[fc], t - frames[fc].first);
+  float raw_d[4], gm[4], w0[4] = {0.f, 0.f, 0.f, 0.f};
+  load_tile(frames[fc], G, t, raw_d, gm, w0);
+
+  for (;;) {
+    const FrameConst &F = frames[fc];
+    const float alpha = F.alpha;
+    float c[4], lin[4], u[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int row = 4 * grp + r;
+      lin[r] = (!PAD || F.geom == NCS_GEOM_REFERENCE || (row < G.r_hi && col < G.c_hi))
+                   ? 1.0f : 0.0f;
+      c[r] = lin[r] * (raw_d[r] + gm[r]);
+      u[r] = raw_d[r];
     }
 
     // starting point, cost and gradient
@@ -553,9 +589,7 @@ void solve_kernel(SolveArgs A) {
         red[0] = ll_lane(u, gm, c, lin, gll);
         red[1] = u[0] * gnc[0] + u[1] * gnc[1] + (u[2] * gnc[2] + u[3] * gnc[3]);
         // warm start: keep it unless the plain start scores a lower cost
-        float w[4], wg[4], wll[4];
-#pragma unroll
-        for (int r = 0; r < 4; r++) w[r] = A.u0[t * kTilePix + (4 * grp + r) * kTile + col];
+        float w[4] = {w0[0], w0[1], w0[2], w0[3]}, wg[4], wll[4];
         apply_k<GEN>(w, T, wg);
         float both[4];
         both[0] = red[0];
@@ -782,6 +816,19 @@ void solve_kernel(SolveArgs A) {
       step = 1.0f;
     }
 
+    // prefetch the next tile: its loads overlap the final evaluation and
+    // stitch of this one
+    const long long tn = next_tile();
+    int fn = fc;
+    TileGeom Gn = G;
+    float nd[4] = {0.f, 0.f, 0.f, 0.f}, ng[4] = {0.f, 0.f, 0.f, 0.f};
+    float nw[4] = {0.f, 0.f, 0.f, 0.f};
+    if (tn >= 0) {
+      fn = frame_of(tn);
+      Gn = tile_geom(frames[fn], tn - frames[fn].first);
+      load_tile(frames[fn], Gn, tn, nd, ng, nw);
+    }
+
     // final cost at the returned point, NC from an exact K u
     if (!gnc_exact) apply_k<GEN>(u, T, gnc);
     {
@@ -806,6 +853,16 @@ void solve_kernel(SolveArgs A) {
       if (A.status) A.status[t] = status;
       if (A.cost) A.cost[t] = f;
       if (A.evaluations) A.evaluations[t] = evals;
+    }
+    if (tn < 0) break;
+    t = tn;
+    fc = fn;
+    G = Gn;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      raw_d[r] = nd[r];
+      gm[r] = ng[r];
+      w0[r] = nw[r];
     }
   }
 }
