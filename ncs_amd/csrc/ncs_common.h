@@ -14,14 +14,14 @@ constexpr int kWave = 64;          // gfx950 wavefront
 constexpr int kWavesPerBlock = 4;  // 256-thread workgroups, one tile per wave
 constexpr int kMaxWavesPerGroup = 4;  // persistent solver: waves per workgroup 1..4
 
-// Work queue of the persistent solver: the tiles are split into kQueues
+// Work queue of the persistent solver: 7/8 of the tiles are assigned to the
+// waves statically in contiguous blocks; the rest are split into kQueues
 // contiguous ranges, one per XCD (workgroups are dispatched round-robin over
-// the 8 XCDs), each with its own counter on its own 128-byte line.  A wave
-// takes chunks of kChunk tiles from its home range, one tile at a time near
-// the end of a range, then steals from the other ranges.
+// the 8 XCDs), each with its own counter on its own 128-byte line, taken
+// kChunk tiles per atomic.
 constexpr int kQueues = 8;
 constexpr int kQueueStride = 32;   // unsigned ints between counters
-constexpr int kChunk = 8;
+constexpr int kChunk = 4;
 constexpr size_t kQueueBytes = kQueues * kQueueStride * sizeof(unsigned int);
 constexpr float kFitMin = 1.0e-6f; // negative-value guard (ncs.cl:45, ncs.c:169)
 

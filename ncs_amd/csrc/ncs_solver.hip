@@ -477,41 +477,46 @@ void solve_kernel(SolveArgs A) {
     }
   }
 
-  // Work queue: chunks of the home range of this workgroup's XCD first, then
-  // of the other ranges.  next_tile() gives the wave's next tile (uniform),
-  // or -1 when every range is exhausted.
-  const int home = blockIdx.x % kQueues;
-  const long long waves_per_queue =
-      ((long long)gridDim.x * (blockDim.x >> 6) + kQueues - 1) / kQueues;
-  const long long tail = 4LL * kChunk * waves_per_queue;
-  int qv = 0;                   // ranges found exhausted
-  long long next = 0, end = 0;  // this wave's current chunk [next, end)
+  // Tile schedule.  Device-scope atomics are slow on this part (they are
+  // resolved beyond the XCD's L2), so most tiles are assigned statically: wave
+  // w owns the contiguous block [w S, w S + S), S = floor(7/8 total / waves).
+  // The remaining tiles are split into kQueues ranges, one per XCD, that
+  // waves draining early take kChunk at a time with one atomic per chunk,
+  // home range first, then the others.  next_tile() gives the wave's next
+  // tile (uniform) or -1 when every range is exhausted.
+  // Tile indices fit 32 bits (the host caps a batch below 2^32 tiles).  The
+  // static block is the wave's first chunk [next, end).
+  const unsigned int total = (unsigned int)A.total_tiles;
+  unsigned int next, end;
+  {
+    const unsigned int n_waves = gridDim.x * (blockDim.x >> 6);
+    const unsigned int per_wave = (total - total / 8) / n_waves;
+    next = (blockIdx.x * (blockDim.x >> 6) + wave) * per_wave;
+    end = next + per_wave;
+  }
+  int qv = 0;  // dynamic ranges found exhausted
   auto next_tile = [&]() -> long long {
     if (next >= end) {
+      const unsigned int n_waves = gridDim.x * (blockDim.x >> 6);
+      const unsigned int dyn0 = (total - total / 8) / n_waves * n_waves;
+      const unsigned int nd = total - dyn0;
       while (qv < kQueues) {
-        const int q = (home + qv) % kQueues;
-        const long long lo = A.total_tiles * q / kQueues;
-        const long long hi = A.total_tiles * (q + 1) / kQueues;
-        unsigned int *ctr = A.queue + q * kQueueStride;
-        unsigned int got = 0, want = 1;
-        if (lane == 0) {
-          const unsigned int seen =
-              __hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          want = (hi - lo - (long long)seen > tail) ? (unsigned int)kChunk : 1u;
-          got = atomicAdd(ctr, want);
-        }
+        const int q = (blockIdx.x + qv) % kQueues;  // home range first
+        const unsigned int lo = dyn0 + (unsigned int)((unsigned long long)nd * q / kQueues);
+        const unsigned int hi = dyn0 + (unsigned int)((unsigned long long)nd * (q + 1) / kQueues);
+        unsigned int got = 0;
+        if (lane == 0) got = atomicAdd(A.queue + q * kQueueStride, (unsigned int)kChunk);
         got = (unsigned int)__builtin_amdgcn_readfirstlane((int)got);
-        want = (unsigned int)__builtin_amdgcn_readfirstlane((int)want);
-        if (lo + (long long)got < hi) {
+        if (got < hi - lo) {
           next = lo + got;
-          end = min(hi, next + (long long)want);
+          end = min(hi, next + (unsigned int)kChunk);
           break;
         }
         qv++;
       }
       if (next >= end) return -1;
     }
-    return next++;
+    return (long long)(next++);
   };
 
   // The frame table is read through the constant address space (scalar
