@@ -50,7 +50,21 @@ def _frame(seed):
     return fr["data"], fr["gamma"]
 
 
-def make_frames(n, pool_size):
+def make_frames(n, pool_size, cache=None):
+    """The batch's frames; with `cache` (an .npz path) they are read from it
+    when present and written to it otherwise (profiler runs read the cache and
+    start no worker processes)."""
+    if cache and os.path.exists(cache):
+        with numpy.load(cache) as z:
+            return [(z["d%d" % i], z["g%d" % i]) for i in range(n)]
+    frames = _make_frames(n, pool_size)
+    if cache:
+        numpy.savez(cache, **{k: v for i, (d, g) in enumerate(frames)
+                              for k, v in (("d%d" % i, d), ("g%d" % i, g))})
+    return frames
+
+
+def _make_frames(n, pool_size):
     if pool_size > 1:
         from multiprocessing import get_context
         with get_context("fork").Pool(pool_size) as pool:
@@ -124,26 +138,45 @@ def run_gpu(frames_np, args, weights):
 
     # schedule: grid and workgroup size tuned on the first frame (history
     # stays m = 5, the workload's)
-    tune = ctx.autotune(desc[:1], _lib.default_params(history=HISTORY, max_iterations=MAX_IT),
-                        tune_history=False)
+    sched = None
+    if args.schedule_cache and os.path.exists(args.schedule_cache):
+        with open(args.schedule_cache) as fh:
+            sched = json.load(fh)
+    if sched is None:
+        tune = ctx.autotune(desc[:1], _lib.default_params(history=HISTORY, max_iterations=MAX_IT),
+                            tune_history=False)
+    else:
+        tune = sched["m5"]
     print("[bench] autotune: %s" % tune, file=sys.stderr, flush=True)
-    plain = _lib.default_params(history=HISTORY, max_iterations=MAX_IT)
+    plain = _lib.default_params(history=HISTORY, max_iterations=MAX_IT,
+                                waves_per_cu=tune["waves_per_cu"],
+                                waves_per_group=tune["waves_per_group"])
     timed(plain, "plain", args.steps, args.warmup)
     results["tune"] = tune
     # liblbfgs's scalar initial inverse Hessian s.y / y.y (ncs.cl:1123)
     scalar = _lib.default_params(history=HISTORY, max_iterations=MAX_IT, precond=0)
     timed(scalar, "plain_scalar_h0", max(1, min(args.steps, 3)), 1)
     # the autotuner also picks the history depth (history=0 below)
-    tune_h = ctx.autotune(desc[:1], _lib.default_params(max_iterations=MAX_IT),
-                          tune_history=True)
+    if sched is None:
+        tune_h = ctx.autotune(desc[:1], _lib.default_params(max_iterations=MAX_IT),
+                              tune_history=True)
+        if args.schedule_cache:
+            with open(args.schedule_cache, "w") as fh:
+                json.dump({"m5": tune, "tuned": tune_h}, fh)
+    else:
+        tune_h = sched["tuned"]
     print("[bench] autotune with depth: %s" % tune_h, file=sys.stderr, flush=True)
     results["tune_history"] = tune_h
-    timed(_lib.default_params(history=0, max_iterations=MAX_IT), "tuned", args.steps,
-          args.warmup)
+    timed(_lib.default_params(history=tune_h["history"], max_iterations=MAX_IT,
+                              waves_per_cu=tune_h["waves_per_cu"],
+                              waves_per_group=tune_h["waves_per_group"]),
+          "tuned", args.steps, args.warmup)
     if weights is not None:
         from ncs_amd import warmstart
         warmstart.set_context_weights(weights)
-        learned = _lib.default_params(history=0, max_iterations=MAX_IT,
+        learned = _lib.default_params(history=tune_h["history"], max_iterations=MAX_IT,
+                                      waves_per_cu=tune_h["waves_per_cu"],
+                                      waves_per_group=tune_h["waves_per_group"],
                                       start=_lib.START_LEARNED)
         timed(learned, "learned", args.steps, args.warmup)
         # the net alone, and the fallback decision on the first frame
@@ -569,6 +602,11 @@ def main():
                     help="run a 1 GiB device copy first (PMC byte-counter calibration)")
     ap.add_argument("--workload", choices=("batch", "uneven", "stream"), default="batch")
     ap.add_argument("--stream-frames", type=int, default=200)
+    ap.add_argument("--frames-cache", default=None,
+                    help="read the batch from this .npz if present, else write it there")
+    ap.add_argument("--schedule-cache", default=None,
+                    help="read the autotuned schedules from this JSON if present (no "
+                         "autotuning), else write them there")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the NCS path shards by frame with no exchange "
@@ -600,7 +638,8 @@ def main():
                                                   log=lambda *a: print("[bench]", *a,
                                                                        file=sys.stderr))
     t0 = time.perf_counter()
-    frames_np = make_frames(args.frames, min(args.frames, 16, os.cpu_count() or 1))
+    frames_np = make_frames(args.frames, min(args.frames, 16, os.cpu_count() or 1),
+                            args.frames_cache)
     print("[bench] generated %d frames in %.1f s" % (len(frames_np), time.perf_counter() - t0),
           file=sys.stderr, flush=True)
 
@@ -717,8 +756,9 @@ def main():
         "learned_warm": learned_info,
         "roofline": (roofline_of(res["tuned"], res["tune_history"]["history"])
                      if start_used == "tuned" else roofline_of(plain, HISTORY)),
-        "tiles_per_s": round(total / plain["wall_s"], 1),
-        "tiles_per_s_nominal_16x16": round(len(frames_np) * (H // 16) * (W // 16) / plain["wall_s"], 1),
+        "tiles_per_s": round(total / candidates[start_used][2], 1),
+        "tiles_per_s_nominal_16x16": round(len(frames_np) * (H // 16) * (W // 16)
+                                           / candidates[start_used][2], 1),
         "iterations": {"mean": round(float(it.mean()), 3), "p50": float(numpy.median(it)),
                        "max": int(it.max())},
         "status": {str(k): int(v) for k, v in zip(*numpy.unique(plain["status"], return_counts=True))},
