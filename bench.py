@@ -72,6 +72,22 @@ def _make_frames(n, pool_size):
     return [_frame(s) for s in range(1, n + 1)]
 
 
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` on this workload from the committed
+    rocprofv3 summaries (FETCH_SIZE and WRITE_SIZE passes calibrated on a
+    device copy), or None."""
+    import re
+    d = os.path.join(ROOT, "profiles")
+    pat = re.compile(r"^traffic_bytes_per_launch (\S+): (\d+)$")
+    for fn in (sorted(os.listdir(d)) if os.path.isdir(d) else []):
+        with open(os.path.join(d, fn)) as fh:
+            for line in fh:
+                m = pat.match(line.strip())
+                if m and m.group(1) == kernel:
+                    return int(m.group(2))
+    return None
+
+
 def flops_per_tile(iters, evals, m, precond=1):
     """Floating-point work per tile of the gfx950 solver, as executed.
 
@@ -673,10 +689,14 @@ def main():
         ev = it + 1  # one accepted trial per iteration plus the start (lower bound)
         flops = float(numpy.sum(flops_per_tile(it, ev, m, precond=1)))
         achieved = flops / (kms * 1e-3) / 1e12
+        kernel = "solve_kernel<%d,false,false>" % m
         return {"bound": "mfma", "achieved": round(achieved, 3),
                 "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
-                "kernel": "solve_kernel<%d,false,false>" % m, "kernel_ms": round(kms, 3),
+                "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+                "traffic": profiled_traffic(kernel),
+                "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0, "
+                                "profiles/)",
+                "kernel": kernel, "kernel_ms": round(kms, 3),
                 "flop_per_launch": flops,
                 "hbm_gbs_algorithmic": round(alg_bytes / (kms * 1e-3) / 1e9, 2)}
 
