@@ -662,7 +662,7 @@ void solve_kernel(SolveArgs A) {
       }
 
       // line search; the first trial is taken at the clamped initial step
-      float gt[4], phi;
+      float gt[4], phi, dphi_acc = 0.0f;  // dphi_acc: phi'(t) of the accepted trial
       float tstep = fminf(fmaxf(step, A.min_step), A.max_step);
       int ls;
       {
@@ -675,6 +675,7 @@ void solve_kernel(SolveArgs A) {
           if (A.max_ls > 1 && tstep != A.max_step && tstep != A.min_step &&
               phi <= A.ftol * dg0 * tstep && fabsf(dphi) <= A.gtol * (-dg0)) {
             ls = LS_ACCEPT;  // what the full search decides on this trial
+            dphi_acc = dphi;
           } else {
             // Full More-Thuente search (about once per tile).  Its state is
             // kept in LDS and the memory clobbers stop the compiler from
@@ -684,6 +685,7 @@ void solve_kernel(SolveArgs A) {
             S_.stmin = 0.0f;
             S_.stmax = 5.0f * tstep;
             ls = mt_post(S_, tstep, phi, dphi, A);
+            dphi_acc = dphi;
             while (ls == LS_CONTINUE) {
               tstep = mt_pre(S_, tstep, A);
               asm volatile("" ::: "memory");
@@ -693,13 +695,18 @@ void solve_kernel(SolveArgs A) {
               phi = red[0] + alpha * tstep * (a1 + 0.5f * tstep * a2);
               const float dp = red[1] + alpha * (a1 + tstep * a2);
               ls = mt_post(S_, tstep, phi, dp, A);
+              dphi_acc = dp;
             }
           }
         } else {
           // Armijo backtracking (ncs.cl:979-1035)
           int trials = 1;
           for (;;) {
-            if (phi <= A.ftol * dg0 * tstep) { ls = LS_ACCEPT; break; }
+            if (phi <= A.ftol * dg0 * tstep) {
+              ls = LS_ACCEPT;
+              dphi_acc = red[1] + alpha * (a1 + tstep * a2);
+              break;
+            }
             tstep *= A.stepm;
             if (!(tstep >= A.min_step)) { ls = LS_FAIL_MINSTEP; break; }
             if (trials >= A.max_ls) { ls = LS_FAIL_PRECISION; break; }
@@ -732,25 +739,27 @@ void solve_kernel(SolveArgs A) {
         gnc_exact = true;
       }
       f += phi;
-      float ys, yy;
+      // y.s = t (phi'(t) - phi'(0)) needs no reduction; the fourth slot is
+      // y.y for the scalar initial Hessian, the mean curvature for the
+      // spectral one
+      const float ys = tstep * (dphi_acc - dg0);
+      float yy = 0.0f;
       for (int pass = 0; pass < 2; pass++) {
 #pragma unroll
         for (int r = 0; r < 4; r++) {
           gnew[r] = fmaf(alpha, gncn[r], gt[r]);
           y[r] = gnew[r] - g[r];
         }
-        float red[5] = {
-            y[0] * s[0] + y[1] * s[1] + (y[2] * s[2] + y[3] * s[3]),
-            y[0] * y[0] + y[1] * y[1] + (y[2] * y[2] + y[3] * y[3]),
+        float red[3] = {
             gnew[0] * gnew[0] + gnew[1] * gnew[1] + (gnew[2] * gnew[2] + gnew[3] * gnew[3]),
             ut[0] * ut[0] + ut[1] * ut[1] + (ut[2] * ut[2] + ut[3] * ut[3]),
-            A.precond ? curv_lane(ut, gm, c) : 0.0f};
-        wave_sum<5>(red);
-        ys = red[0];
-        yy = red[1];
-        gg = red[2];
-        uu = red[3];
-        kappa = red[4] * (1.0f / 256.0f);
+            A.precond ? curv_lane(ut, gm, c)
+                      : y[0] * y[0] + y[1] * y[1] + (y[2] * y[2] + y[3] * y[3])};
+        wave_sum<3>(red);
+        gg = red[0];
+        uu = red[1];
+        kappa = red[2] * (1.0f / 256.0f);
+        yy = red[2];
         done = sqrtf(gg) <= A.eps * fmaxf(1.0f, sqrtf(uu));
         if (!done || gnc_exact) break;
         // confirm convergence with an exact NC gradient
@@ -768,13 +777,14 @@ void solve_kernel(SolveArgs A) {
       if (k == A.max_it) { status = NCS_REACHED_MAXITERS; break; }
 
       // history update: newest pair into the LDS rings
-      if (ys > 0.0f) {
+      const bool added = ys > 0.0f;
+      if (added) {
         head = (head + 1 == M) ? 0 : head + 1;
         Sb[head * 64 + lane] = make_float4(s[0], s[1], s[2], s[3]);
         Yb[head * 64 + lane] = make_float4(y[0], y[1], y[2], y[3]);
 #pragma unroll
         for (int j = M - 1; j > 0; j--) rho[j] = rho[j - 1];
-        rho[0] = 1.0f / ys;
+        rho[0] = rcpf(ys);
         ys_last = ys;
         yy_last = yy;
         nh = min(nh + 1, M);
@@ -790,7 +800,8 @@ void solve_kernel(SolveArgs A) {
           const float4 sv = Sb[slot * 64 + lane];
           const float4 yv = Yb[slot * 64 + lane];
           const float sj[4] = {sv.x, sv.y, sv.z, sv.w};
-          al[j] = rho[j] * wave_dot(sj, dvec);
+          // the pair added in this step: s.(-g) = -t phi'(t), no reduction
+          al[j] = rho[j] * ((j == 0 && added) ? -tstep * dphi_acc : wave_dot(sj, dvec));
           dvec[0] = fmaf(-al[j], yv.x, dvec[0]);
           dvec[1] = fmaf(-al[j], yv.y, dvec[1]);
           dvec[2] = fmaf(-al[j], yv.z, dvec[2]);
@@ -800,7 +811,7 @@ void solve_kernel(SolveArgs A) {
       if (A.precond) {
         apply_h0(dvec, T, kappa, alpha);
       } else {  // liblbfgs: H0 = (s.y / y.y) I
-        const float hscale = (nh > 0) ? ys_last / yy_last : rsqrtf(gg);
+        const float hscale = (nh > 0) ? ys_last * rcpf(yy_last) : rsqrtf(gg);
 #pragma unroll
         for (int r = 0; r < 4; r++) dvec[r] *= hscale;
       }
