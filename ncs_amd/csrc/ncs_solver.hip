@@ -78,82 +78,118 @@ __device__ __forceinline__ float wave_dot(const float (&a)[4], const float (&b)[
 __device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
 
 // log(1 + x) for x > -1, branch-free: the rounding error of u = 1 + x is
-// added back to log(u) to first order, log1p(x) = log(u) + (x - (u - 1)) / u
-// (the correction term of Goldberg's log1p).  With the transcendental unit's
-// log2, measured on gfx950 over |x| in [1e-8, 0.25]: max relative error
-// 2.3e-7, as accurate as a degree-9 atanh series.
+// added back to log(u), log1p(x) = log(u) + (x - (u - 1)) / u (the
+// correction term of Goldberg's log1p), with 1/u taken as 2 - u: the
+// correction is below half an ulp of 1, so the x^2 error of that factor is
+// negligible.  With the transcendental unit's log2, measured on gfx950 over
+// |x| in [1e-8, 0.25] (with 1/u exact): max relative error 2.3e-7, as
+// accurate as a degree-9 atanh series.
 __device__ __forceinline__ float log1p_acc(float x) {
+#ifdef NCS_AB_LOG1P_SERIES
+  if (fabsf(x) < 0.25f) {
+    float z = x * rcpf(2.0f + x);
+    float z2 = z * z;
+    float p = 0.11111111f;
+    p = fmaf(p, z2, 0.14285715f);
+    p = fmaf(p, z2, 0.2f);
+    p = fmaf(p, z2, 0.33333334f);
+    p = fmaf(p, z2, 1.0f);
+    return 2.0f * z * p;
+  }
+  return __logf(1.0f + x);
+#else
   const float u = 1.0f + x;
-  return fmaf(x - (u - 1.0f), rcpf(u), __logf(u));
+  return fmaf(x - (u - 1.0f), 2.0f - u, __logf(u));
+#endif
 }
 
 // ---------------------------------------------------------------------------
 // NC gradient operator on the matrix cores
 // ---------------------------------------------------------------------------
 
-struct MaskTables {
-  float ka[4];  // R[lane%16][4g + r]
-  float kb[4];  // R[4g + r][lane%16]
-  float qe[4];  // 2 * even-even part of q at Z position (lane%16, 4g + r)
-  float qo[4];  // +-2 * odd-odd part, coupling to the partner lane ^ 40
+// Per-lane tables, 16 floats per lane: R[lane%16][4g + r], R[4g + r][lane%16],
+// 2 * the even-even part of q at Z position (lane%16, 4g + r), and +-2 * the
+// odd-odd part coupling to the partner lane ^ 40.
+struct MaskTables {  // held in VGPRs
+  f4 ka_, kb_, qe_, qo_;
+  __device__ f4 ka() const { return ka_; }
+  __device__ f4 kb() const { return kb_; }
+  __device__ f4 qe() const { return qe_; }
+  __device__ f4 qo() const { return qo_; }
+};
+
+// The same tables read from LDS at each use: the solver keeps one copy per
+// workgroup there, which saves 12-16 VGPRs per lane across its main loop.
+struct LdsTables {
+  const f4 *p;  // this lane's 4 entries
+  __device__ f4 ka() const { return p[0]; }
+  __device__ f4 kb() const { return p[1]; }
+  __device__ f4 qe() const { return p[2]; }
+  __device__ f4 qo() const { return p[3]; }
 };
 
 // out = R^T (w * (R x R^T) [+ qo * partner]) R in the tile layout: the NC
 // Hessian product K x for w = qe (plus the odd-odd coupling when GEN), or
 // the spectral initial inverse Hessian for w = 1 / (kappa + alpha qe).
-template <bool ODD>
-__device__ __forceinline__ void rdft_sandwich(const float (&x)[4], const MaskTables &T,
+template <bool ODD, class TB>
+__device__ __forceinline__ void rdft_sandwich(const float (&x)[4], const TB &T,
                                               const float (&wdiag)[4], float (&out)[4]) {
+  const f4 ka = T.ka();
   f4 z1 = {0.f, 0.f, 0.f, 0.f};
   // (R X)^T = X^T R^T : data as the A operand
-  z1 = __builtin_amdgcn_mfma_f32_16x16x4f32(x[0], T.ka[0], z1, 0, 0, 0);
-  z1 = __builtin_amdgcn_mfma_f32_16x16x4f32(x[1], T.ka[1], z1, 0, 0, 0);
-  z1 = __builtin_amdgcn_mfma_f32_16x16x4f32(x[2], T.ka[2], z1, 0, 0, 0);
-  z1 = __builtin_amdgcn_mfma_f32_16x16x4f32(x[3], T.ka[3], z1, 0, 0, 0);
+  z1 = __builtin_amdgcn_mfma_f32_16x16x4f32(x[0], ka[0], z1, 0, 0, 0);
+  z1 = __builtin_amdgcn_mfma_f32_16x16x4f32(x[1], ka[1], z1, 0, 0, 0);
+  z1 = __builtin_amdgcn_mfma_f32_16x16x4f32(x[2], ka[2], z1, 0, 0, 0);
+  z1 = __builtin_amdgcn_mfma_f32_16x16x4f32(x[3], ka[3], z1, 0, 0, 0);
   f4 z = {0.f, 0.f, 0.f, 0.f};
   // Z^T = R (R X)^T : data as the B operand
-  z = __builtin_amdgcn_mfma_f32_16x16x4f32(T.ka[0], z1[0], z, 0, 0, 0);
-  z = __builtin_amdgcn_mfma_f32_16x16x4f32(T.ka[1], z1[1], z, 0, 0, 0);
-  z = __builtin_amdgcn_mfma_f32_16x16x4f32(T.ka[2], z1[2], z, 0, 0, 0);
-  z = __builtin_amdgcn_mfma_f32_16x16x4f32(T.ka[3], z1[3], z, 0, 0, 0);
+  z = __builtin_amdgcn_mfma_f32_16x16x4f32(ka[0], z1[0], z, 0, 0, 0);
+  z = __builtin_amdgcn_mfma_f32_16x16x4f32(ka[1], z1[1], z, 0, 0, 0);
+  z = __builtin_amdgcn_mfma_f32_16x16x4f32(ka[2], z1[2], z, 0, 0, 0);
+  z = __builtin_amdgcn_mfma_f32_16x16x4f32(ka[3], z1[3], z, 0, 0, 0);
   float w[4];
 #pragma unroll
   for (int r = 0; r < 4; r++) w[r] = wdiag[r] * z[r];
   if constexpr (ODD) {
+    const f4 qo = T.qo();
 #pragma unroll
-    for (int r = 0; r < 4; r++) w[r] = fmaf(T.qo[r], __shfl_xor(z[r], 40), w[r]);
+    for (int r = 0; r < 4; r++) w[r] = fmaf(qo[r], __shfl_xor(z[r], 40), w[r]);
   }
+  const f4 kb = T.kb();
   f4 e = {0.f, 0.f, 0.f, 0.f};
   // W R : W^T held, so data as the A operand
-  e = __builtin_amdgcn_mfma_f32_16x16x4f32(w[0], T.kb[0], e, 0, 0, 0);
-  e = __builtin_amdgcn_mfma_f32_16x16x4f32(w[1], T.kb[1], e, 0, 0, 0);
-  e = __builtin_amdgcn_mfma_f32_16x16x4f32(w[2], T.kb[2], e, 0, 0, 0);
-  e = __builtin_amdgcn_mfma_f32_16x16x4f32(w[3], T.kb[3], e, 0, 0, 0);
+  e = __builtin_amdgcn_mfma_f32_16x16x4f32(w[0], kb[0], e, 0, 0, 0);
+  e = __builtin_amdgcn_mfma_f32_16x16x4f32(w[1], kb[1], e, 0, 0, 0);
+  e = __builtin_amdgcn_mfma_f32_16x16x4f32(w[2], kb[2], e, 0, 0, 0);
+  e = __builtin_amdgcn_mfma_f32_16x16x4f32(w[3], kb[3], e, 0, 0, 0);
   f4 g = {0.f, 0.f, 0.f, 0.f};
   // R^T (W R)
-  g = __builtin_amdgcn_mfma_f32_16x16x4f32(T.kb[0], e[0], g, 0, 0, 0);
-  g = __builtin_amdgcn_mfma_f32_16x16x4f32(T.kb[1], e[1], g, 0, 0, 0);
-  g = __builtin_amdgcn_mfma_f32_16x16x4f32(T.kb[2], e[2], g, 0, 0, 0);
-  g = __builtin_amdgcn_mfma_f32_16x16x4f32(T.kb[3], e[3], g, 0, 0, 0);
+  g = __builtin_amdgcn_mfma_f32_16x16x4f32(kb[0], e[0], g, 0, 0, 0);
+  g = __builtin_amdgcn_mfma_f32_16x16x4f32(kb[1], e[1], g, 0, 0, 0);
+  g = __builtin_amdgcn_mfma_f32_16x16x4f32(kb[2], e[2], g, 0, 0, 0);
+  g = __builtin_amdgcn_mfma_f32_16x16x4f32(kb[3], e[3], g, 0, 0, 0);
 #pragma unroll
   for (int r = 0; r < 4; r++) out[r] = g[r];
 }
 
 // gnc = K x where K = Hessian of NC (x, gnc in the tile layout).
-template <bool GEN>
-__device__ __forceinline__ void apply_k(const float (&x)[4], const MaskTables &T,
-                                        float (&out)[4]) {
-  rdft_sandwich<GEN>(x, T, T.qe, out);
+template <bool GEN, class TB>
+__device__ __forceinline__ void apply_k(const float (&x)[4], const TB &T, float (&out)[4]) {
+  const f4 q = T.qe();
+  const float qe[4] = {q[0], q[1], q[2], q[3]};
+  rdft_sandwich<GEN>(x, T, qe, out);
 }
 
 // x <- H0 x with H0 = R^T diag(1 / (kappa + alpha qe)) R: the inverse of the
 // Hessian with the likelihood part replaced by its tile mean kappa and the
 // mask by its even-even part.
-__device__ __forceinline__ void apply_h0(float (&x)[4], const MaskTables &T, float kappa,
+template <class TB>
+__device__ __forceinline__ void apply_h0(float (&x)[4], const TB &T, float kappa,
                                          float alpha) {
+  const f4 q = T.qe();
   float w[4];
 #pragma unroll
-  for (int r = 0; r < 4; r++) w[r] = rcpf(fmaf(alpha, T.qe[r], kappa));
+  for (int r = 0; r < 4; r++) w[r] = rcpf(fmaf(alpha, q[r], kappa));
   float tmp[4] = {x[0], x[1], x[2], x[3]};
   rdft_sandwich<false>(tmp, T, w, x);
 }
@@ -386,18 +422,23 @@ __device__ __forceinline__ float ll_value_lane(const float (&u)[4], const float 
   return s;
 }
 
-// Per-lane likelihood gradient lin - c / max(u + gamma, FITMIN) (ncs.c:158-175).
+// Per-lane likelihood gradient lin - c / max(u + gamma, FITMIN)
+// (ncs.c:158-175); iv receives 1 / max(u + gamma, FITMIN), which the line
+// search and the curvature reuse until u moves.
 __device__ __forceinline__ void ll_grad_lane(const float (&u)[4], const float (&gm)[4],
                                              const float (&c)[4], const float (&lin)[4],
-                                             float (&gll)[4]) {
+                                             float (&gll)[4], float (&iv)[4]) {
 #pragma unroll
-  for (int r = 0; r < 4; r++) gll[r] = lin[r] - c[r] * rcpf(fmaxf(u[r] + gm[r], kFitMin));
+  for (int r = 0; r < 4; r++) {
+    iv[r] = rcpf(fmaxf(u[r] + gm[r], kFitMin));
+    gll[r] = lin[r] - c[r] * iv[r];
+  }
 }
 
 __device__ __forceinline__ float ll_lane(const float (&u)[4], const float (&gm)[4],
                                          const float (&c)[4], const float (&lin)[4],
-                                         float (&gll)[4]) {
-  ll_grad_lane(u, gm, c, lin, gll);
+                                         float (&gll)[4], float (&iv)[4]) {
+  ll_grad_lane(u, gm, c, lin, gll, iv);
   return ll_value_lane(u, gm, c, lin);
 }
 
@@ -405,23 +446,23 @@ __device__ __forceinline__ float ll_lane(const float (&u)[4], const float (&gm)[
 // phi(t) = f(u + t d) - f(u) and of phi'(t), and the likelihood gradient at
 // u + t d.  The NC part is the quadratic t a1 + t^2 a2 / 2 (added by the
 // caller).
-// Per-lane sum of the likelihood curvature c / max(u + gamma, FITMIN)^2.
-__device__ __forceinline__ float curv_lane(const float (&u)[4], const float (&gm)[4],
-                                           const float (&c)[4]) {
+// Per-lane sum of the likelihood curvature c / max(u + gamma, FITMIN)^2,
+// from iv = 1 / max(u + gamma, FITMIN).
+__device__ __forceinline__ float curv_lane(const float (&c)[4], const float (&iv)[4]) {
   float s = 0.0f;
 #pragma unroll
-  for (int r = 0; r < 4; r++) {
-    const float rv = rcpf(fmaxf(u[r] + gm[r], kFitMin));
-    s = fmaf(c[r] * rv, rv, s);
-  }
+  for (int r = 0; r < 4; r++) s = fmaf(c[r] * iv[r], iv[r], s);
   return s;
 }
 
-template <bool PAD>
+// iv = 1 / max(u + gamma, FITMIN) at u (recomputed here unless CACHED:
+// keeping it across iterations costs four VGPRs, which only short histories
+// can spare); ivt receives it at u + t d.
+template <bool PAD, bool CACHED>
 __device__ __forceinline__ void ll_trial(float t, const float (&u)[4], const float (&dv)[4],
                                          const float (&gm)[4], const float (&c)[4],
-                                         const float (&lin)[4], float (&gt)[4],
-                                         float (&red)[2]) {
+                                         const float (&lin)[4], const float (&iv)[4],
+                                         float (&gt)[4], float (&ivt)[4], float (&red)[2]) {
   red[0] = 0.0f;
   red[1] = 0.0f;
 #pragma unroll
@@ -429,14 +470,14 @@ __device__ __forceinline__ void ll_trial(float t, const float (&u)[4], const flo
     const float td = t * dv[r];
     const float w0 = u[r] + gm[r];
     const float w = w0 + td;
-    const float v0 = fmaxf(w0, kFitMin);
     const float vt = fmaxf(w, kFitMin);
     // relative change of max(u + gamma, FITMIN); exact step t d when no guard
-    const float delta = (w0 >= kFitMin && w >= kFitMin) ? td : vt - v0;
-    const float lg = log1p_acc(delta * rcpf(v0));
+    const float delta = (w0 >= kFitMin && w >= kFitMin) ? td : vt - fmaxf(w0, kFitMin);
+    const float lg = log1p_acc(delta * (CACHED ? iv[r] : rcpf(fmaxf(w0, kFitMin))));
     const float l = PAD ? lin[r] : 1.0f;
     red[0] += l * td - c[r] * lg;
-    gt[r] = l - c[r] * rcpf(vt);
+    ivt[r] = rcpf(vt);
+    gt[r] = l - c[r] * ivt[r];
     red[1] += gt[r] * dv[r];
   }
   wave_sum<2>(red);
@@ -448,29 +489,34 @@ __device__ __forceinline__ void ll_trial(float t, const float (&u)[4], const flo
 // lane reads another lane's LDS and no barrier is needed).  PAD enables
 // zero-padded pixels (block geometry).
 template <int M, bool GEN, bool PAD>
-__global__ __launch_bounds__(64 * kMaxWavesPerGroup) __attribute__((amdgpu_waves_per_eu(4, 8)))
+#ifndef NCS_AB_WPE_MIN
+#define NCS_AB_WPE_MIN 4
+#endif
+#ifndef NCS_AB_DYN_SHIFT
+#define NCS_AB_DYN_SHIFT 3
+#endif
+__global__ __launch_bounds__(64 * kMaxWavesPerGroup) __attribute__((amdgpu_waves_per_eu(NCS_AB_WPE_MIN, 8)))
 void solve_kernel(SolveArgs A) {
   extern __shared__ float4 lds_hist[];
   __shared__ MtState mt_state[kMaxWavesPerGroup];  // line-search state per wave
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform
+  constexpr bool kCacheIv = M <= 3;
   MtState &mt_lds = mt_state[wave];
   float4 *const Sb = lds_hist + wave * (2 * M * 64);  // s vectors, ring of M
   float4 *const Yb = Sb + M * 64;                     // y vectors, ring of M
   const int lane = threadIdx.x & 63;
   const int col = lane & 15, grp = lane >> 4;
 
-  MaskTables T;
+  // OTF / DFT tables: each wave writes its lanes' entries (the same values
+  // for every wave of the group) and each lane reads back only its own, so
+  // no barrier is needed.
+  __shared__ f4 tab_lds[kWave * 4];
   {
-    const float4 *tb = reinterpret_cast<const float4 *>(A.tables + lane * 16);
-    float4 t0 = tb[0], t1 = tb[1], t2 = tb[2];
-    T.ka[0] = t0.x; T.ka[1] = t0.y; T.ka[2] = t0.z; T.ka[3] = t0.w;
-    T.kb[0] = t1.x; T.kb[1] = t1.y; T.kb[2] = t1.z; T.kb[3] = t1.w;
-    T.qe[0] = t2.x; T.qe[1] = t2.y; T.qe[2] = t2.z; T.qe[3] = t2.w;
-    if constexpr (GEN) {
-      float4 t3 = tb[3];
-      T.qo[0] = t3.x; T.qo[1] = t3.y; T.qo[2] = t3.z; T.qo[3] = t3.w;
-    }
+    const f4 *tb = reinterpret_cast<const f4 *>(A.tables + lane * 16);
+#pragma unroll
+    for (int i = 0; i < 4; i++) tab_lds[lane * 4 + i] = tb[i];
   }
+  const LdsTables T{tab_lds + lane * 4};
 
   // Tile schedule.  Device-scope atomics are slow on this part (they are
   // resolved beyond the XCD's L2), so most tiles are assigned statically: wave
@@ -485,7 +531,7 @@ void solve_kernel(SolveArgs A) {
   unsigned int next, end;
   {
     const unsigned int n_waves = gridDim.x * (blockDim.x >> 6);
-    const unsigned int per_wave = (total - total / 8) / n_waves;
+    const unsigned int per_wave = (total - (total >> NCS_AB_DYN_SHIFT)) / n_waves;
     next = (blockIdx.x * (blockDim.x >> 6) + wave) * per_wave;
     end = next + per_wave;
   }
@@ -493,7 +539,7 @@ void solve_kernel(SolveArgs A) {
   auto next_tile = [&]() -> long long {
     if (next >= end) {
       const unsigned int n_waves = gridDim.x * (blockDim.x >> 6);
-      const unsigned int dyn0 = (total - total / 8) / n_waves * n_waves;
+      const unsigned int dyn0 = (total - (total >> NCS_AB_DYN_SHIFT)) / n_waves * n_waves;
       const unsigned int nd = total - dyn0;
       while (qv < kQueues) {
         const int q = (blockIdx.x + qv) % kQueues;  // home range first
@@ -578,30 +624,35 @@ void solve_kernel(SolveArgs A) {
     }
 
     // starting point, cost and gradient
-    float gnc[4], gll[4], g[4];
+    float gnc[4], gll[4], g[4], iv[4];
     apply_k<GEN>(u, T, gnc);
     float f = 0.0f;  // f(u) - f(u_start): only differences drive the search
     {
       float red[2];
       if (A.u0 == nullptr) {
-        ll_grad_lane(u, gm, c, lin, gll);
+        ll_grad_lane(u, gm, c, lin, gll, iv);
       } else {
-        red[0] = ll_lane(u, gm, c, lin, gll);
+        red[0] = ll_lane(u, gm, c, lin, gll, iv);
         red[1] = u[0] * gnc[0] + u[1] * gnc[1] + (u[2] * gnc[2] + u[3] * gnc[3]);
         // warm start: keep it unless the plain start scores a lower cost
-        float w[4] = {w0[0], w0[1], w0[2], w0[3]}, wg[4], wll[4];
+        float w[4] = {w0[0], w0[1], w0[2], w0[3]}, wg[4], wll[4], wiv[4];
         apply_k<GEN>(w, T, wg);
         float both[4];
         both[0] = red[0];
         both[1] = red[1];
-        both[2] = ll_lane(w, gm, c, lin, wll);
+        both[2] = ll_lane(w, gm, c, lin, wll, wiv);
         both[3] = w[0] * wg[0] + w[1] * wg[1] + (w[2] * wg[2] + w[3] * wg[3]);
         wave_sum<4>(both);
         const float f_plain = both[0] + 0.5f * alpha * both[1];
         const float f_warm = both[2] + 0.5f * alpha * both[3];
         if (!A.fallback || f_warm < f_plain || !(f_plain == f_plain)) {
 #pragma unroll
-          for (int r = 0; r < 4; r++) { u[r] = w[r]; gnc[r] = wg[r]; gll[r] = wll[r]; }
+          for (int r = 0; r < 4; r++) {
+            u[r] = w[r];
+            gnc[r] = wg[r];
+            gll[r] = wll[r];
+            iv[r] = wiv[r];
+          }
         }
       }
     }
@@ -614,7 +665,7 @@ void solve_kernel(SolveArgs A) {
     {
       float red[3] = {g[0] * g[0] + g[1] * g[1] + (g[2] * g[2] + g[3] * g[3]),
                       u[0] * u[0] + u[1] * u[1] + (u[2] * u[2] + u[3] * u[3]),
-                      curv_lane(u, gm, c)};
+                      curv_lane(c, iv)};
       wave_sum<3>(red);
       gg = red[0];
       uu = red[1];
@@ -671,12 +722,12 @@ void solve_kernel(SolveArgs A) {
       }
 
       // line search; the first trial is taken at the clamped initial step
-      float gt[4], phi, dphi_acc = 0.0f;  // dphi_acc: phi'(t) of the accepted trial
+      float gt[4], ivt[4], phi, dphi_acc = 0.0f;  // dphi_acc: phi'(t) of the accepted trial
       float tstep = fminf(fmaxf(step, A.min_step), A.max_step);
       int ls;
       {
         float red[2];
-        ll_trial<PAD>(tstep, u, dvec, gm, c, lin, gt, red);
+        ll_trial<PAD, kCacheIv>(tstep, u, dvec, gm, c, lin, iv, gt, ivt, red);
         evals++;
         phi = red[0] + alpha * tstep * (a1 + 0.5f * tstep * a2);
         const float dphi = red[1] + alpha * (a1 + tstep * a2);
@@ -698,7 +749,7 @@ void solve_kernel(SolveArgs A) {
             while (ls == LS_CONTINUE) {
               tstep = mt_pre(S_, tstep, A);
               asm volatile("" ::: "memory");
-              ll_trial<PAD>(tstep, u, dvec, gm, c, lin, gt, red);
+              ll_trial<PAD, kCacheIv>(tstep, u, dvec, gm, c, lin, iv, gt, ivt, red);
               asm volatile("" ::: "memory");
               evals++;
               phi = red[0] + alpha * tstep * (a1 + 0.5f * tstep * a2);
@@ -719,7 +770,7 @@ void solve_kernel(SolveArgs A) {
             tstep *= A.stepm;
             if (!(tstep >= A.min_step)) { ls = LS_FAIL_MINSTEP; break; }
             if (trials >= A.max_ls) { ls = LS_FAIL_PRECISION; break; }
-            ll_trial<PAD>(tstep, u, dvec, gm, c, lin, gt, red);
+            ll_trial<PAD, kCacheIv>(tstep, u, dvec, gm, c, lin, iv, gt, ivt, red);
             evals++;
             trials++;
             phi = red[0] + alpha * tstep * (a1 + 0.5f * tstep * a2);
@@ -762,7 +813,7 @@ void solve_kernel(SolveArgs A) {
         float red[3] = {
             gnew[0] * gnew[0] + gnew[1] * gnew[1] + (gnew[2] * gnew[2] + gnew[3] * gnew[3]),
             ut[0] * ut[0] + ut[1] * ut[1] + (ut[2] * ut[2] + ut[3] * ut[3]),
-            A.precond ? curv_lane(ut, gm, c)
+            A.precond ? curv_lane(c, ivt)
                       : y[0] * y[0] + y[1] * y[1] + (y[2] * y[2] + y[3] * y[3])};
         wave_sum<3>(red);
         gg = red[0];
@@ -780,6 +831,7 @@ void solve_kernel(SolveArgs A) {
         u[r] = ut[r];
         g[r] = gnew[r];
         gnc[r] = gncn[r];
+        iv[r] = ivt[r];
       }
       if (done) { status = NCS_SUCCESS; break; }
       if (!moved) { status = NCS_REACHED_MAXPRECISION; break; }
@@ -904,11 +956,12 @@ __global__ __launch_bounds__(256) void eval_kernel(const float *data, const floa
   const long long t = (long long)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
   if (t >= n) return;  // whole waves leave together
   MaskTables T;
-  for (int r = 0; r < 4; r++) {
-    T.ka[r] = tables[lane * 16 + r];
-    T.kb[r] = tables[lane * 16 + 4 + r];
-    T.qe[r] = tables[lane * 16 + 8 + r];
-    T.qo[r] = tables[lane * 16 + 12 + r];
+  {
+    const f4 *tb = reinterpret_cast<const f4 *>(tables + lane * 16);
+    T.ka_ = tb[0];
+    T.kb_ = tb[1];
+    T.qe_ = tb[2];
+    T.qo_ = tb[3];
   }
   float u[4], gm[4], c[4], lin[4], gnc[4], gll[4];
   for (int r = 0; r < 4; r++) {
@@ -920,7 +973,8 @@ __global__ __launch_bounds__(256) void eval_kernel(const float *data, const floa
   }
   apply_k<GEN>(u, T, gnc);
   float red[2];
-  red[0] = ll_lane(u, gm, c, lin, gll);
+  float iv[4];
+  red[0] = ll_lane(u, gm, c, lin, gll, iv);
   red[1] = u[0] * gnc[0] + u[1] * gnc[1] + (u[2] * gnc[2] + u[3] * gnc[3]);
   wave_sum<2>(red);
   if (lane == 0 && cost) cost[t] = red[0] + 0.5f * alpha * red[1];
