@@ -128,12 +128,11 @@ struct LdsTables {
   __device__ f4 qo() const { return p[3]; }
 };
 
-// out = R^T (w * (R x R^T) [+ qo * partner]) R in the tile layout: the NC
-// Hessian product K x for w = qe (plus the odd-odd coupling when GEN), or
-// the spectral initial inverse Hessian for w = 1 / (kappa + alpha qe).
-template <bool ODD, class TB>
-__device__ __forceinline__ void rdft_sandwich(const float (&x)[4], const TB &T,
-                                              const float (&wdiag)[4], float (&out)[4]) {
+// Real-DFT coefficients of a tile: Z^T = (R X R^T)^T in the MFMA
+// accumulator layout (the transform is orthonormal, so dot products are
+// the same in either basis).  Two GEMM stages, 8 MFMA 16x16x4.
+template <class TB>
+__device__ __forceinline__ void to_spec(const float (&x)[4], const TB &T, float (&z)[4]) {
   const f4 ka = T.ka();
   f4 z1 = {0.f, 0.f, 0.f, 0.f};
   // (R X)^T = X^T R^T : data as the A operand
@@ -141,20 +140,19 @@ __device__ __forceinline__ void rdft_sandwich(const float (&x)[4], const TB &T,
   z1 = __builtin_amdgcn_mfma_f32_16x16x4f32(x[1], ka[1], z1, 0, 0, 0);
   z1 = __builtin_amdgcn_mfma_f32_16x16x4f32(x[2], ka[2], z1, 0, 0, 0);
   z1 = __builtin_amdgcn_mfma_f32_16x16x4f32(x[3], ka[3], z1, 0, 0, 0);
-  f4 z = {0.f, 0.f, 0.f, 0.f};
+  f4 zz = {0.f, 0.f, 0.f, 0.f};
   // Z^T = R (R X)^T : data as the B operand
-  z = __builtin_amdgcn_mfma_f32_16x16x4f32(ka[0], z1[0], z, 0, 0, 0);
-  z = __builtin_amdgcn_mfma_f32_16x16x4f32(ka[1], z1[1], z, 0, 0, 0);
-  z = __builtin_amdgcn_mfma_f32_16x16x4f32(ka[2], z1[2], z, 0, 0, 0);
-  z = __builtin_amdgcn_mfma_f32_16x16x4f32(ka[3], z1[3], z, 0, 0, 0);
-  float w[4];
+  zz = __builtin_amdgcn_mfma_f32_16x16x4f32(ka[0], z1[0], zz, 0, 0, 0);
+  zz = __builtin_amdgcn_mfma_f32_16x16x4f32(ka[1], z1[1], zz, 0, 0, 0);
+  zz = __builtin_amdgcn_mfma_f32_16x16x4f32(ka[2], z1[2], zz, 0, 0, 0);
+  zz = __builtin_amdgcn_mfma_f32_16x16x4f32(ka[3], z1[3], zz, 0, 0, 0);
 #pragma unroll
-  for (int r = 0; r < 4; r++) w[r] = wdiag[r] * z[r];
-  if constexpr (ODD) {
-    const f4 qo = T.qo();
-#pragma unroll
-    for (int r = 0; r < 4; r++) w[r] = fmaf(qo[r], __shfl_xor(z[r], 40), w[r]);
-  }
+  for (int r = 0; r < 4; r++) z[r] = zz[r];
+}
+
+// Inverse of to_spec: X = R^T W R back in the tile layout.  8 MFMA.
+template <class TB>
+__device__ __forceinline__ void to_pix(const float (&w)[4], const TB &T, float (&x)[4]) {
   const f4 kb = T.kb();
   f4 e = {0.f, 0.f, 0.f, 0.f};
   // W R : W^T held, so data as the A operand
@@ -169,29 +167,43 @@ __device__ __forceinline__ void rdft_sandwich(const float (&x)[4], const TB &T,
   g = __builtin_amdgcn_mfma_f32_16x16x4f32(kb[2], e[2], g, 0, 0, 0);
   g = __builtin_amdgcn_mfma_f32_16x16x4f32(kb[3], e[3], g, 0, 0, 0);
 #pragma unroll
-  for (int r = 0; r < 4; r++) out[r] = g[r];
+  for (int r = 0; r < 4; r++) x[r] = g[r];
 }
 
-// gnc = K x where K = Hessian of NC (x, gnc in the tile layout).
+// The NC Hessian K in the spectral basis: the diagonal 2 q_ee plus, for a
+// mask with an odd-odd part (GEN), the coupling to the partner lane ^ 40.
 template <bool GEN, class TB>
-__device__ __forceinline__ void apply_k(const float (&x)[4], const TB &T, float (&out)[4]) {
-  const f4 q = T.qe();
-  const float qe[4] = {q[0], q[1], q[2], q[3]};
-  rdft_sandwich<GEN>(x, T, qe, out);
-}
-
-// x <- H0 x with H0 = R^T diag(1 / (kappa + alpha qe)) R: the inverse of the
-// Hessian with the likelihood part replaced by its tile mean kappa and the
-// mask by its even-even part.
-template <class TB>
-__device__ __forceinline__ void apply_h0(float (&x)[4], const TB &T, float kappa,
-                                         float alpha) {
-  const f4 q = T.qe();
+__device__ __forceinline__ void nc_spec(const float (&z)[4], const TB &T, float (&out)[4]) {
+  const f4 qe = T.qe();
   float w[4];
 #pragma unroll
-  for (int r = 0; r < 4; r++) w[r] = rcpf(fmaf(alpha, q[r], kappa));
-  float tmp[4] = {x[0], x[1], x[2], x[3]};
-  rdft_sandwich<false>(tmp, T, w, x);
+  for (int r = 0; r < 4; r++) w[r] = qe[r] * z[r];
+  if constexpr (GEN) {
+    const f4 qo = T.qo();
+#pragma unroll
+    for (int r = 0; r < 4; r++) w[r] = fmaf(qo[r], __shfl_xor(z[r], 40), w[r]);
+  }
+#pragma unroll
+  for (int r = 0; r < 4; r++) out[r] = w[r];
+}
+
+// gnc = K x in the tile layout (the evaluation kernel's NC gradient).
+template <bool GEN, class TB>
+__device__ __forceinline__ void apply_k(const float (&x)[4], const TB &T, float (&out)[4]) {
+  float z[4], w[4];
+  to_spec(x, T, z);
+  nc_spec<GEN>(z, T, w);
+  to_pix(w, T, out);
+}
+
+// x <- H0 x in the spectral basis, H0 = diag(1 / (kappa + alpha qe)): the
+// inverse of the Hessian with the likelihood part replaced by its tile mean
+// kappa and the mask by its even-even part.
+template <class TB>
+__device__ __forceinline__ void h0_spec(float (&x)[4], const TB &T, float kappa, float alpha) {
+  const f4 q = T.qe();
+#pragma unroll
+  for (int r = 0; r < 4; r++) x[r] *= rcpf(fmaf(alpha, q[r], kappa));
 }
 
 // ---------------------------------------------------------------------------
@@ -623,25 +635,29 @@ void solve_kernel(SolveArgs A) {
       u[r] = raw_d[r];
     }
 
-    // starting point, cost and gradient
-    float gnc[4], gll[4], g[4], iv[4];
-    apply_k<GEN>(u, T, gnc);
+    // Start.  The L-BFGS state (gradient, NC gradient, direction, history)
+    // lives in the orthonormal real-DFT basis, where K and the spectral H0
+    // are diagonal: each iteration needs one transform to pixels (the
+    // direction, for the likelihood trial) and one back (the likelihood
+    // gradient), 8 MFMA each.  Dot products are basis-independent.
+    float gs[4], gncs[4], zs[4], iv[4];  // spectral g, K u; pixel 1/(u+gamma)
     float f = 0.0f;  // f(u) - f(u_start): only differences drive the search
     {
-      float red[2];
+      float gll[4];
+      to_spec(u, T, zs);
+      nc_spec<GEN>(zs, T, gncs);
       if (A.u0 == nullptr) {
         ll_grad_lane(u, gm, c, lin, gll, iv);
       } else {
-        red[0] = ll_lane(u, gm, c, lin, gll, iv);
-        red[1] = u[0] * gnc[0] + u[1] * gnc[1] + (u[2] * gnc[2] + u[3] * gnc[3]);
         // warm start: keep it unless the plain start scores a lower cost
-        float w[4] = {w0[0], w0[1], w0[2], w0[3]}, wg[4], wll[4], wiv[4];
-        apply_k<GEN>(w, T, wg);
         float both[4];
-        both[0] = red[0];
-        both[1] = red[1];
+        both[0] = ll_lane(u, gm, c, lin, gll, iv);
+        both[1] = zs[0] * gncs[0] + zs[1] * gncs[1] + (zs[2] * gncs[2] + zs[3] * gncs[3]);
+        float w[4] = {w0[0], w0[1], w0[2], w0[3]}, wz[4], wgs[4], wll[4], wiv[4];
+        to_spec(w, T, wz);
+        nc_spec<GEN>(wz, T, wgs);
         both[2] = ll_lane(w, gm, c, lin, wll, wiv);
-        both[3] = w[0] * wg[0] + w[1] * wg[1] + (w[2] * wg[2] + w[3] * wg[3]);
+        both[3] = wz[0] * wgs[0] + wz[1] * wgs[1] + (wz[2] * wgs[2] + wz[3] * wgs[3]);
         wave_sum<4>(both);
         const float f_plain = both[0] + 0.5f * alpha * both[1];
         const float f_warm = both[2] + 0.5f * alpha * both[3];
@@ -649,21 +665,23 @@ void solve_kernel(SolveArgs A) {
 #pragma unroll
           for (int r = 0; r < 4; r++) {
             u[r] = w[r];
-            gnc[r] = wg[r];
+            gncs[r] = wgs[r];
             gll[r] = wll[r];
             iv[r] = wiv[r];
           }
         }
       }
-    }
+      float gls[4];
+      to_spec(gll, T, gls);
 #pragma unroll
-    for (int r = 0; r < 4; r++) g[r] = fmaf(alpha, gnc[r], gll[r]);
+      for (int r = 0; r < 4; r++) gs[r] = fmaf(alpha, gncs[r], gls[r]);
+    }
 
     int status = NCS_REACHED_MAXITERS, iters = 0, evals = 1;
     bool gnc_exact = true;
     float gg, uu, kappa;
     {
-      float red[3] = {g[0] * g[0] + g[1] * g[1] + (g[2] * g[2] + g[3] * g[3]),
+      float red[3] = {gs[0] * gs[0] + gs[1] * gs[1] + (gs[2] * gs[2] + gs[3] * gs[3]),
                       u[0] * u[0] + u[1] * u[1] + (u[2] * u[2] + u[3] * u[3]),
                       curv_lane(c, iv)};
       wave_sum<3>(red);
@@ -671,17 +689,17 @@ void solve_kernel(SolveArgs A) {
       uu = red[1];
       kappa = red[2] * (1.0f / 256.0f);
     }
-    bool done = sqrtf(gg) <= A.eps * fmaxf(1.0f, sqrtf(uu));
+    bool done = gg <= A.eps * A.eps * fmaxf(1.0f, uu);
     if (done) status = NCS_SUCCESS;
 
     float rho[M], al[M];  // rho[j], al[j]: j-th newest pair
     int nh = 0, head = M - 1;
-    float dvec[4];
+    float ds[4];  // search direction, spectral
 #pragma unroll
-    for (int r = 0; r < 4; r++) dvec[r] = -g[r];
+    for (int r = 0; r < 4; r++) ds[r] = -gs[r];
     float step = rsqrtf(gg);
     if (A.precond && !done) {
-      apply_h0(dvec, T, kappa, alpha);
+      h0_spec(ds, T, kappa, alpha);
       step = 1.0f;
     }
     float ys_last = 1.0f, yy_last = 1.0f;
@@ -690,14 +708,14 @@ void solve_kernel(SolveArgs A) {
       // K d and the quadratic coefficients of NC along d.  A direction that
       // is not a descent direction (rounding at the FP32 noise floor can
       // spoil the history) is replaced once by the history-free one, -H0 g.
-      float kd[4];
+      float kds[4], dp[4];  // K d (spectral), d (pixels)
       float dg0, a1, a2;
       for (int attempt = 0;; attempt++) {
-        apply_k<GEN>(dvec, T, kd);
+        nc_spec<GEN>(ds, T, kds);
         float red[3] = {
-            g[0] * dvec[0] + g[1] * dvec[1] + (g[2] * dvec[2] + g[3] * dvec[3]),
-            gnc[0] * dvec[0] + gnc[1] * dvec[1] + (gnc[2] * dvec[2] + gnc[3] * dvec[3]),
-            kd[0] * dvec[0] + kd[1] * dvec[1] + (kd[2] * dvec[2] + kd[3] * dvec[3])};
+            gs[0] * ds[0] + gs[1] * ds[1] + (gs[2] * ds[2] + gs[3] * ds[3]),
+            gncs[0] * ds[0] + gncs[1] * ds[1] + (gncs[2] * ds[2] + gncs[3] * ds[3]),
+            kds[0] * ds[0] + kds[1] * ds[1] + (kds[2] * ds[2] + kds[3] * ds[3])};
         wave_sum<3>(red);
         dg0 = red[0];
         a1 = red[1];
@@ -705,13 +723,13 @@ void solve_kernel(SolveArgs A) {
         if (dg0 <= 0.0f || attempt == 1 || nh == 0) break;
         nh = 0;
 #pragma unroll
-        for (int r = 0; r < 4; r++) dvec[r] = -g[r];
+        for (int r = 0; r < 4; r++) ds[r] = -gs[r];
         if (A.precond) {
-          apply_h0(dvec, T, kappa, alpha);
+          h0_spec(ds, T, kappa, alpha);
         } else {
           const float sc = rsqrtf(gg);
 #pragma unroll
-          for (int r = 0; r < 4; r++) dvec[r] *= sc;
+          for (int r = 0; r < 4; r++) ds[r] *= sc;
         }
         step = 1.0f;
       }
@@ -720,6 +738,7 @@ void solve_kernel(SolveArgs A) {
         status = NCS_INCREASING_GRADIENT;
         break;
       }
+      to_pix(ds, T, dp);
 
       // line search; the first trial is taken at the clamped initial step
       float gt[4], ivt[4], phi, dphi_acc = 0.0f;  // dphi_acc: phi'(t) of the accepted trial
@@ -727,7 +746,7 @@ void solve_kernel(SolveArgs A) {
       int ls;
       {
         float red[2];
-        ll_trial<PAD, kCacheIv>(tstep, u, dvec, gm, c, lin, iv, gt, ivt, red);
+        ll_trial<PAD, kCacheIv>(tstep, u, dp, gm, c, lin, iv, gt, ivt, red);
         evals++;
         phi = red[0] + alpha * tstep * (a1 + 0.5f * tstep * a2);
         const float dphi = red[1] + alpha * (a1 + tstep * a2);
@@ -749,13 +768,13 @@ void solve_kernel(SolveArgs A) {
             while (ls == LS_CONTINUE) {
               tstep = mt_pre(S_, tstep, A);
               asm volatile("" ::: "memory");
-              ll_trial<PAD, kCacheIv>(tstep, u, dvec, gm, c, lin, iv, gt, ivt, red);
+              ll_trial<PAD, kCacheIv>(tstep, u, dp, gm, c, lin, iv, gt, ivt, red);
               asm volatile("" ::: "memory");
               evals++;
               phi = red[0] + alpha * tstep * (a1 + 0.5f * tstep * a2);
-              const float dp = red[1] + alpha * (a1 + tstep * a2);
-              ls = mt_post(S_, tstep, phi, dp, A);
-              dphi_acc = dp;
+              const float dpt = red[1] + alpha * (a1 + tstep * a2);
+              ls = mt_post(S_, tstep, phi, dpt, A);
+              dphi_acc = dpt;
             }
           }
         } else {
@@ -770,7 +789,7 @@ void solve_kernel(SolveArgs A) {
             tstep *= A.stepm;
             if (!(tstep >= A.min_step)) { ls = LS_FAIL_MINSTEP; break; }
             if (trials >= A.max_ls) { ls = LS_FAIL_PRECISION; break; }
-            ll_trial<PAD, kCacheIv>(tstep, u, dvec, gm, c, lin, iv, gt, ivt, red);
+            ll_trial<PAD, kCacheIv>(tstep, u, dp, gm, c, lin, iv, gt, ivt, red);
             evals++;
             trials++;
             phi = red[0] + alpha * tstep * (a1 + 0.5f * tstep * a2);
@@ -785,64 +804,71 @@ void solve_kernel(SolveArgs A) {
       }
 
       // accept u + t d
-      float ut[4], s[4], gnew[4], gncn[4], y[4];
+      float ut[4], gnews[4], gncn[4], gts[4];
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        ut[r] = u[r] + tstep * dvec[r];
-        s[r] = ut[r] - u[r];
-        gncn[r] = fmaf(tstep, kd[r], gnc[r]);
+        ut[r] = u[r] + tstep * dp[r];
+        gncn[r] = fmaf(tstep, kds[r], gncs[r]);
       }
-      const bool moved = __any((s[0] != 0.0f) | (s[1] != 0.0f) | (s[2] != 0.0f) | (s[3] != 0.0f));
+      const bool moved = __any((ut[0] != u[0]) | (ut[1] != u[1]) | (ut[2] != u[2]) |
+                               (ut[3] != u[3]));
       gnc_exact = false;
       if ((k & 15) == 0) {  // periodic exact refresh of the NC gradient
-        apply_k<GEN>(ut, T, gncn);
+        to_spec(ut, T, zs);
+        nc_spec<GEN>(zs, T, gncn);
         gnc_exact = true;
       }
+      to_spec(gt, T, gts);
       f += phi;
-      // y.s = t (phi'(t) - phi'(0)) needs no reduction; the fourth slot is
+      // y.s = t (phi'(t) - phi'(0)) needs no reduction; the third slot is
       // y.y for the scalar initial Hessian, the mean curvature for the
       // spectral one
       const float ys = tstep * (dphi_acc - dg0);
       float yy = 0.0f;
       for (int pass = 0; pass < 2; pass++) {
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-          gnew[r] = fmaf(alpha, gncn[r], gt[r]);
-          y[r] = gnew[r] - g[r];
-        }
+        for (int r = 0; r < 4; r++) gnews[r] = fmaf(alpha, gncn[r], gts[r]);
         float red[3] = {
-            gnew[0] * gnew[0] + gnew[1] * gnew[1] + (gnew[2] * gnew[2] + gnew[3] * gnew[3]),
+            gnews[0] * gnews[0] + gnews[1] * gnews[1] + (gnews[2] * gnews[2] + gnews[3] * gnews[3]),
             ut[0] * ut[0] + ut[1] * ut[1] + (ut[2] * ut[2] + ut[3] * ut[3]),
             A.precond ? curv_lane(c, ivt)
-                      : y[0] * y[0] + y[1] * y[1] + (y[2] * y[2] + y[3] * y[3])};
+                      : ((gnews[0] - gs[0]) * (gnews[0] - gs[0]) +
+                         (gnews[1] - gs[1]) * (gnews[1] - gs[1]) +
+                         ((gnews[2] - gs[2]) * (gnews[2] - gs[2]) +
+                          (gnews[3] - gs[3]) * (gnews[3] - gs[3])))};
         wave_sum<3>(red);
         gg = red[0];
         uu = red[1];
         kappa = red[2] * (1.0f / 256.0f);
         yy = red[2];
-        done = sqrtf(gg) <= A.eps * fmaxf(1.0f, sqrtf(uu));
+        done = gg <= A.eps * A.eps * fmaxf(1.0f, uu);
         if (!done || gnc_exact) break;
         // confirm convergence with an exact NC gradient
-        apply_k<GEN>(ut, T, gncn);
+        to_spec(ut, T, zs);
+        nc_spec<GEN>(zs, T, gncn);
         gnc_exact = true;
+      }
+      // history pair: s = t d, y = g_new - g (spectral)
+      const bool added = ys > 0.0f && !done && moved && k != A.max_it;
+      if (added) {
+        head = (head + 1 == M) ? 0 : head + 1;
+        Sb[head * 64 + lane] = make_float4(tstep * ds[0], tstep * ds[1], tstep * ds[2],
+                                           tstep * ds[3]);
+        Yb[head * 64 + lane] = make_float4(gnews[0] - gs[0], gnews[1] - gs[1],
+                                           gnews[2] - gs[2], gnews[3] - gs[3]);
       }
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         u[r] = ut[r];
-        g[r] = gnew[r];
-        gnc[r] = gncn[r];
+        gs[r] = gnews[r];
+        gncs[r] = gncn[r];
         iv[r] = ivt[r];
       }
       if (done) { status = NCS_SUCCESS; break; }
       if (!moved) { status = NCS_REACHED_MAXPRECISION; break; }
       if (k == A.max_it) { status = NCS_REACHED_MAXITERS; break; }
 
-      // history update: newest pair into the LDS rings
-      const bool added = ys > 0.0f;
       if (added) {
-        head = (head + 1 == M) ? 0 : head + 1;
-        Sb[head * 64 + lane] = make_float4(s[0], s[1], s[2], s[3]);
-        Yb[head * 64 + lane] = make_float4(y[0], y[1], y[2], y[3]);
 #pragma unroll
         for (int j = M - 1; j > 0; j--) rho[j] = rho[j - 1];
         rho[0] = rcpf(ys);
@@ -851,9 +877,9 @@ void solve_kernel(SolveArgs A) {
         nh = min(nh + 1, M);
       }
 
-      // two-loop recursion, newest to oldest then back
+      // two-loop recursion in the spectral basis, newest to oldest then back
 #pragma unroll
-      for (int r = 0; r < 4; r++) dvec[r] = -g[r];
+      for (int r = 0; r < 4; r++) ds[r] = -gs[r];
 #pragma unroll
       for (int j = 0; j < M; j++) {
         if (j < nh) {
@@ -862,19 +888,19 @@ void solve_kernel(SolveArgs A) {
           const float4 yv = Yb[slot * 64 + lane];
           const float sj[4] = {sv.x, sv.y, sv.z, sv.w};
           // the pair added in this step: s.(-g) = -t phi'(t), no reduction
-          al[j] = rho[j] * ((j == 0 && added) ? -tstep * dphi_acc : wave_dot(sj, dvec));
-          dvec[0] = fmaf(-al[j], yv.x, dvec[0]);
-          dvec[1] = fmaf(-al[j], yv.y, dvec[1]);
-          dvec[2] = fmaf(-al[j], yv.z, dvec[2]);
-          dvec[3] = fmaf(-al[j], yv.w, dvec[3]);
+          al[j] = rho[j] * ((j == 0 && added) ? -tstep * dphi_acc : wave_dot(sj, ds));
+          ds[0] = fmaf(-al[j], yv.x, ds[0]);
+          ds[1] = fmaf(-al[j], yv.y, ds[1]);
+          ds[2] = fmaf(-al[j], yv.z, ds[2]);
+          ds[3] = fmaf(-al[j], yv.w, ds[3]);
         }
       }
       if (A.precond) {
-        apply_h0(dvec, T, kappa, alpha);
+        h0_spec(ds, T, kappa, alpha);
       } else {  // liblbfgs: H0 = (s.y / y.y) I
         const float hscale = (nh > 0) ? ys_last * rcpf(yy_last) : rsqrtf(gg);
 #pragma unroll
-        for (int r = 0; r < 4; r++) dvec[r] *= hscale;
+        for (int r = 0; r < 4; r++) ds[r] *= hscale;
       }
 #pragma unroll
       for (int j = M - 1; j >= 0; j--) {
@@ -883,11 +909,11 @@ void solve_kernel(SolveArgs A) {
           const float4 sv = Sb[slot * 64 + lane];
           const float4 yv = Yb[slot * 64 + lane];
           const float yj[4] = {yv.x, yv.y, yv.z, yv.w};
-          const float b = al[j] - rho[j] * wave_dot(yj, dvec);
-          dvec[0] = fmaf(b, sv.x, dvec[0]);
-          dvec[1] = fmaf(b, sv.y, dvec[1]);
-          dvec[2] = fmaf(b, sv.z, dvec[2]);
-          dvec[3] = fmaf(b, sv.w, dvec[3]);
+          const float b = al[j] - rho[j] * wave_dot(yj, ds);
+          ds[0] = fmaf(b, sv.x, ds[0]);
+          ds[1] = fmaf(b, sv.y, ds[1]);
+          ds[2] = fmaf(b, sv.z, ds[2]);
+          ds[3] = fmaf(b, sv.w, ds[3]);
         }
       }
       step = 1.0f;
@@ -906,12 +932,13 @@ void solve_kernel(SolveArgs A) {
       load_tile(frames[fn], Gn, tn, nd, ng, nw);
     }
 
-    // final cost at the returned point, NC from an exact K u
-    if (!gnc_exact) apply_k<GEN>(u, T, gnc);
+    // final cost at the returned point, NC from an exact transform of u
     {
+      to_spec(u, T, zs);
+      nc_spec<GEN>(zs, T, gncs);
       float red[2];
       red[0] = ll_value_lane(u, gm, c, lin);
-      red[1] = u[0] * gnc[0] + u[1] * gnc[1] + (u[2] * gnc[2] + u[3] * gnc[3]);
+      red[1] = zs[0] * gncs[0] + zs[1] * gncs[1] + (zs[2] * gncs[2] + zs[3] * gncs[3]);
       wave_sum<2>(red);
       f = red[0] + 0.5f * alpha * red[1];
     }
