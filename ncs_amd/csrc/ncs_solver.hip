@@ -507,12 +507,16 @@ template <int M, bool GEN, bool PAD>
 #ifndef NCS_AB_DYN_SHIFT
 #define NCS_AB_DYN_SHIFT 3
 #endif
-__global__ __launch_bounds__(64 * kMaxWavesPerGroup) __attribute__((amdgpu_waves_per_eu(NCS_AB_WPE_MIN, 8)))
+__global__ __launch_bounds__(64 * kMaxWavesPerGroup)
+__attribute__((amdgpu_waves_per_eu(M <= 1 ? NCS_AB_WPE_MIN + 1 : NCS_AB_WPE_MIN, 8)))
 void solve_kernel(SolveArgs A) {
   extern __shared__ float4 lds_hist[];
   __shared__ MtState mt_state[kMaxWavesPerGroup];  // line-search state per wave
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform
-  constexpr bool kCacheIv = M <= 3;
+#ifndef NCS_AB_CACHE_M
+#define NCS_AB_CACHE_M 3
+#endif
+  constexpr bool kCacheIv = M <= NCS_AB_CACHE_M && M >= 1;
   MtState &mt_lds = mt_state[wave];
   float4 *const Sb = lds_hist + wave * (2 * M * 64);  // s vectors, ring of M
   float4 *const Yb = Sb + M * 64;                     // y vectors, ring of M
@@ -708,9 +712,10 @@ void solve_kernel(SolveArgs A) {
       // K d and the quadratic coefficients of NC along d.  A direction that
       // is not a descent direction (rounding at the FP32 noise floor can
       // spoil the history) is replaced once by the history-free one, -H0 g.
-      float kds[4], dp[4];  // K d (spectral), d (pixels)
+      float dp[4];  // d in pixels
       float dg0, a1, a2;
       for (int attempt = 0;; attempt++) {
+        float kds[4];  // K d (spectral), recomputed at the accepted step
         nc_spec<GEN>(ds, T, kds);
         float red[3] = {
             gs[0] * ds[0] + gs[1] * ds[1] + (gs[2] * ds[2] + gs[3] * ds[3]),
@@ -805,10 +810,11 @@ void solve_kernel(SolveArgs A) {
 
       // accept u + t d
       float ut[4], gnews[4], gncn[4], gts[4];
+      nc_spec<GEN>(ds, T, gncn);
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         ut[r] = u[r] + tstep * dp[r];
-        gncn[r] = fmaf(tstep, kds[r], gncs[r]);
+        gncn[r] = fmaf(tstep, gncn[r], gncs[r]);
       }
       const bool moved = __any((ut[0] != u[0]) | (ut[1] != u[1]) | (ut[2] != u[2]) |
                                (ut[3] != u[3]));
