@@ -72,13 +72,14 @@ def _make_frames(n, pool_size):
     return [_frame(s) for s in range(1, n + 1)]
 
 
-def profiled_traffic(kernel):
-    """HBM bytes per launch of `kernel` on this workload from the committed
-    rocprofv3 summaries (FETCH_SIZE and WRITE_SIZE passes calibrated on a
-    device copy), or None."""
+def profiled(key, kernel):
+    """A per-launch figure of `kernel` on this workload from the committed
+    rocprofv3 summaries, or None: "traffic_bytes_per_launch" (FETCH_SIZE and
+    WRITE_SIZE passes calibrated on a device copy) or
+    "mfma_instructions_per_launch" (SQ_INSTS_MFMA)."""
     import re
     d = os.path.join(ROOT, "profiles")
-    pat = re.compile(r"^traffic_bytes_per_launch (\S+): (\d+)$")
+    pat = re.compile(r"^%s (\S+): (\d+)$" % re.escape(key))
     for fn in (sorted(os.listdir(d)) if os.path.isdir(d) else []):
         with open(os.path.join(d, fn)) as fh:
             for line in fh:
@@ -88,19 +89,23 @@ def profiled_traffic(kernel):
     return None
 
 
-def flops_per_tile(iters, evals, m, precond=1):
-    """Floating-point work per tile of the gfx950 solver, as executed.
+def flops_per_tile(iters, trials, m):
+    """Algorithmic floating-point work per tile, as defined for this path by
+    the DFT-matrix formulation of the cost (per evaluation: the forward
+    U = F X F^T of a real tile, 6 real 16^3 GEMMs = 49,152 flop, its adjoint
+    49,152 flop, and 17 flop per pixel for the noise sum, spectrum, log,
+    likelihood gradient and combination: F_eval = 102,656), plus 2 flop per
+    pixel per line-search trial and (13 + 8m) flop per pixel per L-BFGS
+    iteration (norms, dots, two-loop recursion):
 
-    Real-DFT sandwich (the NC Hessian product K x, and with precond the
-    spectral initial inverse Hessian H0 x): 16 MFMA 16x16x4 = 32,768 flop.
-    K is applied 2 + I + floor(I/16) times (start, one per iteration,
-    periodic refresh, convergence confirm), H0 1 + I times (first direction,
-    one per iteration); likelihood trials 20 flop per pixel (incl. log1p,
-    reciprocal); iteration vector work (8m + 25) flop per pixel (two-loop
-    recursion, dots, updates)."""
-    n = 256
-    applies = 2 + iters + iters // 16 + (precond * (1 + iters))
-    return 32768.0 * applies + 20.0 * n * evals + (8.0 * m + 25.0) * n * iters
+        F_tile = F_eval (1 + T) + 512 T + (13 + 8m) 256 I
+
+    with I iterations and T trials.  The solver executes less than this (the
+    NC Hessian is diagonal in its real-DFT basis: 8 MFMA per basis change,
+    two changes per iteration); the executed MFMA count is reported
+    separately from the profiler."""
+    f_eval = 102656.0
+    return f_eval * (1.0 + trials) + 512.0 * trials + (13.0 + 8.0 * m) * 256.0 * iters
 
 
 def run_gpu(frames_np, args, weights):
@@ -685,18 +690,25 @@ def main():
 
     def roofline_of(run, m):
         kms = run["kernel_ms"]
-        it = run["iters"].astype(numpy.int64)
-        ev = it + 1  # one accepted trial per iteration plus the start (lower bound)
-        flops = float(numpy.sum(flops_per_tile(it, ev, m, precond=1)))
+        it = run["iters"].astype(numpy.float64)
+        # trials T = I: one trial per iteration, a lower bound (the first
+        # trial is accepted on almost every iteration)
+        flops = float(numpy.sum(flops_per_tile(it, it, m)))
         achieved = flops / (kms * 1e-3) / 1e12
         kernel = "solve_kernel<%d,false,false>" % m
         return {"bound": "mfma", "achieved": round(achieved, 3),
                 "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
-                "traffic": profiled_traffic(kernel),
+                "traffic": profiled("traffic_bytes_per_launch", kernel),
                 "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0, "
                                 "profiles/)",
+                "flop_model": "algorithmic: F_eval (1 + T) + 512 T + (13 + 8m) 256 I per tile, "
+                              "F_eval = 102,656 (DFT-matrix evaluation), T = I",
                 "kernel": kernel, "kernel_ms": round(kms, 3),
+                "executed_mfma_tflops": (
+                    round(profiled("mfma_instructions_per_launch", kernel) * 2048.0
+                          / (kms * 1e-3) / 1e12, 2)
+                    if profiled("mfma_instructions_per_launch", kernel) else None),
                 "flop_per_launch": flops,
                 "hbm_gbs_algorithmic": round(alg_bytes / (kms * 1e-3) / 1e9, 2)}
 
