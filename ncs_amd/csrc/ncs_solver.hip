@@ -85,22 +85,8 @@ __device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x)
 // |x| in [1e-8, 0.25] (with 1/u exact): max relative error 2.3e-7, as
 // accurate as a degree-9 atanh series.
 __device__ __forceinline__ float log1p_acc(float x) {
-#ifdef NCS_AB_LOG1P_SERIES
-  if (fabsf(x) < 0.25f) {
-    float z = x * rcpf(2.0f + x);
-    float z2 = z * z;
-    float p = 0.11111111f;
-    p = fmaf(p, z2, 0.14285715f);
-    p = fmaf(p, z2, 0.2f);
-    p = fmaf(p, z2, 0.33333334f);
-    p = fmaf(p, z2, 1.0f);
-    return 2.0f * z * p;
-  }
-  return __logf(1.0f + x);
-#else
   const float u = 1.0f + x;
   return fmaf(x - (u - 1.0f), 2.0f - u, __logf(u));
-#endif
 }
 
 // ---------------------------------------------------------------------------
@@ -500,23 +486,19 @@ __device__ __forceinline__ void ll_trial(float t, const float (&u)[4], const flo
 // (each lane keeps its own 4 values at float4 index slot * 64 + lane, so no
 // lane reads another lane's LDS and no barrier is needed).  PAD enables
 // zero-padded pixels (block geometry).
+//
+// Occupancy target: 5 waves per SIMD (at most 96 VGPRs) for histories up to
+// 4, 4 waves (128 VGPRs) above; the latency of the reduction chains and of
+// the MFMA stages is hidden by other waves, so the extra wave pays
+// (measured: -10 % at history 1).
 template <int M, bool GEN, bool PAD>
-#ifndef NCS_AB_WPE_MIN
-#define NCS_AB_WPE_MIN 4
-#endif
-#ifndef NCS_AB_DYN_SHIFT
-#define NCS_AB_DYN_SHIFT 3
-#endif
 __global__ __launch_bounds__(64 * kMaxWavesPerGroup)
-__attribute__((amdgpu_waves_per_eu(M <= 1 ? NCS_AB_WPE_MIN + 1 : NCS_AB_WPE_MIN, 8)))
+__attribute__((amdgpu_waves_per_eu(M <= 4 ? 5 : 4, 8)))
 void solve_kernel(SolveArgs A) {
   extern __shared__ float4 lds_hist[];
   __shared__ MtState mt_state[kMaxWavesPerGroup];  // line-search state per wave
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform
-#ifndef NCS_AB_CACHE_M
-#define NCS_AB_CACHE_M 3
-#endif
-  constexpr bool kCacheIv = M <= NCS_AB_CACHE_M && M >= 1;
+  constexpr bool kCacheIv = M <= 3;  // keep 1/(u+gamma) across iterations
   MtState &mt_lds = mt_state[wave];
   float4 *const Sb = lds_hist + wave * (2 * M * 64);  // s vectors, ring of M
   float4 *const Yb = Sb + M * 64;                     // y vectors, ring of M
@@ -547,7 +529,7 @@ void solve_kernel(SolveArgs A) {
   unsigned int next, end;
   {
     const unsigned int n_waves = gridDim.x * (blockDim.x >> 6);
-    const unsigned int per_wave = (total - (total >> NCS_AB_DYN_SHIFT)) / n_waves;
+    const unsigned int per_wave = (total - total / 8) / n_waves;
     next = (blockIdx.x * (blockDim.x >> 6) + wave) * per_wave;
     end = next + per_wave;
   }
@@ -555,7 +537,7 @@ void solve_kernel(SolveArgs A) {
   auto next_tile = [&]() -> long long {
     if (next >= end) {
       const unsigned int n_waves = gridDim.x * (blockDim.x >> 6);
-      const unsigned int dyn0 = (total - (total >> NCS_AB_DYN_SHIFT)) / n_waves * n_waves;
+      const unsigned int dyn0 = (total - total / 8) / n_waves * n_waves;
       const unsigned int nd = total - dyn0;
       while (qv < kQueues) {
         const int q = (blockIdx.x + qv) % kQueues;  // home range first
@@ -597,6 +579,11 @@ void solve_kernel(SolveArgs A) {
   // for blocks) and, for warm starts, its start tile.
   auto load_tile = [&](const FrameConst &F, const TileGeom &G, long long t, float (&dd)[4],
                        float (&gg)[4], float (&w0)[4]) {
+    // lane-derived offsets are recomputed per tile: hoisted out of the tile
+    // loop they would be spilled, and reloading them costs more
+    int ln = lane;
+    asm volatile("" : "+v"(ln));
+    const int col = ln & 15, grp = ln >> 4;
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       const int row = 4 * grp + r;
@@ -950,6 +937,9 @@ void solve_kernel(SolveArgs A) {
     }
 
     // stitch the owned interior and per-tile results
+    int ln = lane;
+    asm volatile("" : "+v"(ln));
+    const int col = ln & 15, grp = ln >> 4;
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       const int row = 4 * grp + r;
