@@ -224,3 +224,37 @@ def test_autotune_picks_a_schedule_that_is_used(otf):
     rel = numpy.abs(cost["tuned"] - cost["m5"]) / numpy.abs(cost["m5"])
     _say("tuned vs history 5: max rel cost diff", rel.max())
     assert rel.max() <= 1e-5
+
+
+def test_pipeline_learned_start_matches_eager_learned(otf):
+    """The net's start inside the captured graph gives what the eager
+    device entry gives with NCS_START_LEARNED, bit for bit, and the costs
+    match the plain start's within 1e-5."""
+    import torch
+    from ncs_amd import warmstart
+    ctx = otf
+    warmstart.set_context_weights(warmstart.load())
+    pin = _Pinned()
+    shape = (96, 140)
+    f = sim.make_frame(71, shape)
+    p_l = _lib.default_params(start=_lib.START_LEARNED)
+    pl = _lib.Pipeline(ctx, [shape], [0.1], params=p_l, keep_tile_results=True)
+    pl.set_gamma(0, f["gamma"])
+    src = pin(shape, numpy.float32)
+    src[...] = f["data"]
+    out = pin(shape, numpy.float32)
+    pl.launch([src], [out])
+    pl.wait()
+    it_p, st_p, co_p = pl.tile_results()
+    pl.close()
+    frames, keep = _device_frames([f["data"]], [f["gamma"]], [0.1])
+    n = _lib.tile_count(*shape)
+    cost = torch.zeros(n, dtype=torch.float32, device="cuda:0")
+    ctx.reduce_frames_device(frames, p_l, cost_ptr=cost.data_ptr())
+    torch.cuda.synchronize()
+    assert numpy.array_equal(out, keep[2].cpu().numpy())
+    assert numpy.array_equal(co_p, cost.cpu().numpy())
+    plain = _eager(ctx, [f["data"]], [f["gamma"]], [0.1], _lib.default_params())
+    rel = numpy.abs(co_p.astype(numpy.float64) - plain[3]) / numpy.abs(plain[3])
+    _say("pipeline learned vs plain: max rel cost", rel.max(), "iters", it_p.mean(), plain[1].mean())
+    assert rel.max() <= 1e-5 and numpy.all(st_p == 0)
