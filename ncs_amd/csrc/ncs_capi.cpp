@@ -303,6 +303,9 @@ static int build_frames(const ncs_frame *frames, int n, std::vector<FrameDev> &f
     d.gs = f.gamma_stride > 0 ? f.gamma_stride : f.width;
     d.os = f.out_stride > 0 ? f.out_stride : f.width;
     if (d.ds < d.W || d.gs < d.W || d.os < d.W) return fail(NCS_E_ARG, "stride < width");
+    const long long rows = d.H;
+    if (rows * std::max(d.ds, std::max(d.gs, d.os)) >= (1LL << 31))
+      return fail(NCS_E_ARG, "frame " + std::to_string(i) + " spans 2^31 or more elements");
     d.alpha = f.alpha;
     d.geom = f.geometry;
     d.ty = (int)tiles_axis(f.width, f.geometry);
@@ -928,6 +931,8 @@ extern "C" int ncs_hip_pipeline_create(ncs_ctx *ctx, const ncs_pipeline_frame *f
     long long nt = ncs_hip_tile_count(f.height, f.width, f.geometry);
     if (nt < 0) return fail(NCS_E_ARG, "frame " + std::to_string(i) + ": bad size or geometry");
     if (!std::isfinite(f.alpha)) return fail(NCS_E_ARG, "non-finite alpha");
+    if ((long long)f.height * f.width >= (1LL << 31))
+      return fail(NCS_E_ARG, "frame " + std::to_string(i) + " spans 2^31 or more elements");
     pl->off[i] = pl->pix;
     pl->pix += (frame_pixels(f) + 63) / 64 * 64;  // 256-byte aligned frames
     FrameDev &d = fd[i];

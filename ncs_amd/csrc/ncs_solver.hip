@@ -584,25 +584,28 @@ void solve_kernel(SolveArgs A) {
     int ln = lane;
     asm volatile("" : "+v"(ln));
     const int col = ln & 15, grp = ln >> 4;
+    // 32-bit element offsets from uniform base pointers (the host keeps
+    // every frame below 2^31 elements)
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       const int row = 4 * grp + r;
       if (!PAD || F.geom == NCS_GEOM_REFERENCE) {
-        const int gr = min(max(G.bx + row, 0), F.H - 1);
-        const int gc = min(max(G.by + col, 0), F.W - 1);
-        dd[r] = F.data[(long long)gr * F.ds + gc];
-        gg[r] = F.gamma[(long long)gr * F.gs + gc];
+        const unsigned int gr = (unsigned int)min(max(G.bx + row, 0), F.H - 1);
+        const unsigned int gc = (unsigned int)min(max(G.by + col, 0), F.W - 1);
+        dd[r] = F.data[gr * (unsigned int)F.ds + gc];
+        gg[r] = F.gamma[gr * (unsigned int)F.gs + gc];
       } else {
         const bool ok = (row < G.r_hi) && (col < G.c_hi);
-        const int gr = ok ? G.bx + row : 0, gc = ok ? G.by + col : 0;
-        dd[r] = ok ? F.data[(long long)gr * F.ds + gc] : 0.0f;
-        gg[r] = ok ? F.gamma[(long long)gr * F.gs + gc] : 0.0f;
+        const unsigned int gr = ok ? (unsigned int)(G.bx + row) : 0u;
+        const unsigned int gc = ok ? (unsigned int)(G.by + col) : 0u;
+        dd[r] = ok ? F.data[gr * (unsigned int)F.ds + gc] : 0.0f;
+        gg[r] = ok ? F.gamma[gr * (unsigned int)F.gs + gc] : 0.0f;
       }
     }
     if (A.u0 != nullptr) {
-      const float *w = A.u0 + t * kTilePix + (4 * grp) * kTile + col;
+      const float *w = A.u0 + t * kTilePix;  // uniform
 #pragma unroll
-      for (int r = 0; r < 4; r++) w0[r] = w[r * kTile];
+      for (int r = 0; r < 4; r++) w0[r] = w[(unsigned int)((4 * grp + r) * kTile + col)];
     }
   };
 
@@ -925,8 +928,9 @@ void solve_kernel(SolveArgs A) {
       load_tile(frames[fn], Gn, tn, nd, ng, nw);
     }
 
-    // final cost at the returned point, NC from an exact transform of u
-    {
+    // final cost at the returned point (when asked for), NC from an exact
+    // transform of u
+    if (A.cost != nullptr) {
       to_spec(u, T, zs);
       nc_spec<GEN>(zs, T, gncs);
       float red[2];
@@ -940,13 +944,14 @@ void solve_kernel(SolveArgs A) {
     int ln = lane;
     asm volatile("" : "+v"(ln));
     const int col = ln & 15, grp = ln >> 4;
+    float *const ut_base = A.u_tiles != nullptr ? A.u_tiles + t * kTilePix : nullptr;
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       const int row = 4 * grp + r;
       if (F.out != nullptr && row >= G.r_lo && row < G.r_hi && col >= G.c_lo &&
           col < G.c_hi)
-        F.out[(long long)(G.bx + row) * F.os + (G.by + col)] = u[r];
-      if (A.u_tiles != nullptr) A.u_tiles[t * kTilePix + row * kTile + col] = u[r];
+        F.out[(unsigned int)(G.bx + row) * (unsigned int)F.os + (unsigned int)(G.by + col)] = u[r];
+      if (ut_base != nullptr) ut_base[(unsigned int)(row * kTile + col)] = u[r];
     }
     if (lane == 0) {
       if (A.iterations) A.iterations[t] = iters;
