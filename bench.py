@@ -89,7 +89,7 @@ def profiled(key, kernel):
     return None
 
 
-def flops_per_tile(iters, trials, m):
+def flops_per_tile(iters, trials, m, warm=False):
     """Algorithmic floating-point work per tile, as defined for this path by
     the DFT-matrix formulation of the cost (per evaluation: the forward
     U = F X F^T of a real tile, 6 real 16^3 GEMMs = 49,152 flop, its adjoint
@@ -105,10 +105,12 @@ def flops_per_tile(iters, trials, m):
     two changes per iteration); the executed MFMA count is reported
     separately from the profiler."""
     f_eval = 102656.0
-    return f_eval * (1.0 + trials) + 512.0 * trials + (13.0 + 8.0 * m) * 256.0 * iters
+    # a warm start scores both starts: one more evaluation
+    return (f_eval * (1.0 + trials + (1.0 if warm else 0.0)) + 512.0 * trials
+            + (13.0 + 8.0 * m) * 256.0 * iters)
 
 
-def run_gpu(frames_np, args, weights):
+def run_gpu(frames_np, args, weights, lin_coef=None):
     import torch
     from ncs_amd import _lib, masks
 
@@ -192,6 +194,14 @@ def run_gpu(frames_np, args, weights):
                               waves_per_cu=tune_h["waves_per_cu"],
                               waves_per_group=tune_h["waves_per_group"]),
           "tuned", args.steps, args.warmup)
+    if lin_coef is not None:
+        # learned linear start, fused into the solve kernel
+        ctx.set_linear_start(lin_coef)
+        timed(_lib.default_params(history=tune_h["history"], max_iterations=MAX_IT,
+                                  waves_per_cu=tune_h["waves_per_cu"],
+                                  waves_per_group=tune_h["waves_per_group"],
+                                  start=_lib.START_LINEAR),
+              "linear", args.steps, args.warmup)
     if weights is not None:
         from ncs_amd import warmstart
         warmstart.set_context_weights(weights)
@@ -483,7 +493,7 @@ def stream_series(n, shape, seed, dev, blink_p=0.3, segment_um=1.0):
     return adu_np, owner, offset, gain, var
 
 
-def run_stream(args, weights):
+def run_stream(args, weights, lin_coef=None):
     import torch
     from ncs_amd import _lib, masks
     n = args.stream_frames
@@ -499,6 +509,9 @@ def run_stream(args, weights):
     off32, gain32, var32 = (x.astype(numpy.float32) for x in (offset, gain, var))
     results = {}
     modes = [("plain", _lib.START_PLAIN)]
+    if lin_coef is not None:
+        ctx.set_linear_start(lin_coef)
+        modes.append(("linear", _lib.START_LINEAR))
     if weights is not None:
         from ncs_amd import warmstart
         warmstart.set_context_weights(weights)
@@ -563,9 +576,9 @@ def run_stream(args, weights):
                                      float(numpy.max(numpy.abs(r["kept"][k] - o))))
             e["bit_exact_plain"] = e["bit_exact_plain"] and bool(numpy.array_equal(r["kept"][k], o))
     best = "plain"
-    if "learned" in results and par["learned"]["max_rel_cost"] <= 1e-5 and \
-            results["learned"]["wall"] < results["plain"]["wall"]:
-        best = "learned"
+    for label in results:
+        if par[label]["max_rel_cost"] <= 1e-5 and results[label]["wall"] < results[best]["wall"]:
+            best = label
     pix = n * shape[0] * shape[1]
 
     def summary(label):
@@ -644,7 +657,12 @@ def main():
             from ncs_amd import warmstart
             if os.path.exists(warmstart.DEFAULT_WEIGHTS):
                 weights = warmstart.load()
-        run_stream(args, weights)
+        lin_coef = None
+        if not args.no_learned:
+            from ncs_amd import linstart
+            if os.path.exists(linstart.DEFAULT_COEF):
+                lin_coef = linstart.load()
+        run_stream(args, weights, lin_coef)
         return
 
     # learned warm start weights (trained on the GPU by ncs_amd.warmstart);
@@ -681,19 +699,26 @@ def main():
                                log=lambda *a: print("[bench]", *a, file=sys.stderr))
         weights = p.cpu().numpy()
         weights_note = "trained in this run (50k pairs, 1500 steps): no stored weights found"
-    res, ntile_frame, total, devinfo = run_gpu(frames_np, args, weights)
+    lin_coef, lin_note = None, None
+    if not args.no_learned:
+        from ncs_amd import linstart
+        if os.path.exists(linstart.DEFAULT_COEF):
+            lin_coef = linstart.load()
+            lin_note = ("ncs_amd/weights/%s (fitted in closed form on MI355X solver pairs)"
+                        % os.path.basename(linstart.DEFAULT_COEF))
+    res, ntile_frame, total, devinfo = run_gpu(frames_np, args, weights, lin_coef)
     plain = res["plain"]
     pix = len(frames_np) * H * W
     mpix = pix / plain["wall_s"] / 1e6
     alg_bytes = pix * 12.0  # data + gamma in, image out (float32)
     it = plain["iters"].astype(numpy.int64)
 
-    def roofline_of(run, m):
+    def roofline_of(run, m, warm=False):
         kms = run["kernel_ms"]
         it = run["iters"].astype(numpy.float64)
         # trials T = I: one trial per iteration, a lower bound (the first
         # trial is accepted on almost every iteration)
-        flops = float(numpy.sum(flops_per_tile(it, it, m)))
+        flops = float(numpy.sum(flops_per_tile(it, it, m, warm)))
         achieved = flops / (kms * 1e-3) / 1e12
         kernel = "solve_kernel<%d,false,false>" % m
         return {"bound": "mfma", "achieved": round(achieved, 3),
@@ -730,6 +755,24 @@ def main():
                       "cost_vs_plain_max_rel": float(rel.max()),
                       "matched_1e5": bool(rel.max() <= 1e-5)}
         candidates["tuned"] = (tm, tuned_info["matched_1e5"], T["wall_s"])
+    linear_info = None
+    if "linear" in res:
+        L = res["linear"]
+        lm = pix / L["wall_s"] / 1e6
+        rel = numpy.abs(L["cost"].astype(numpy.float64) - plain["cost"]) / numpy.abs(plain["cost"])
+        ref_it = res["tuned"]["iters"] if "tuned" in res else plain["iters"]
+        linear_info = {
+            "value": round(lm, 2), "unit": "Mpix/s", "ms_per_step": round(L["wall_s"] * 1e3, 3),
+            "kernel_ms": round(L["kernel_ms"], 3),
+            "history": res["tune_history"]["history"],
+            "iterations_mean": round(float(L["iters"].mean()), 3),
+            "iterations_saved_vs_same_history_plain": round(float(ref_it.mean() - L["iters"].mean()), 3),
+            "cost_vs_plain_max_rel": float(rel.max()),
+            "cost_vs_plain_p99_rel": float(numpy.percentile(rel, 99)),
+            "matched_1e5": bool(rel.max() <= 1e-5),
+            "coefficients": lin_note,
+            "status": {str(k): int(v) for k, v in zip(*numpy.unique(L["status"], return_counts=True))}}
+        candidates["linear"] = (lm, linear_info["matched_1e5"], L["wall_s"])
     learned_info = None
     if "learned" in res:
         L = res["learned"]
@@ -758,6 +801,13 @@ def main():
     for name, (v, ok, _) in candidates.items():
         if ok and v > value:
             value, start_used = v, name
+
+    def headline_roofline():
+        if start_used in ("tuned", "linear"):
+            return roofline_of(res[start_used], res["tune_history"]["history"],
+                               warm=start_used == "linear")
+        # the learned conv-net path runs two kernels; its solve is reported by the plain one
+        return roofline_of(plain, HISTORY)
     line = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -785,9 +835,9 @@ def main():
         "plain_scalar_h0": _scalar_summary(res, plain, pix),
         "tuned": tuned_info,
         "autotune": res.get("tune"),
+        "linear_start": linear_info,
         "learned_warm": learned_info,
-        "roofline": (roofline_of(res["tuned"], res["tune_history"]["history"])
-                     if start_used == "tuned" else roofline_of(plain, HISTORY)),
+        "roofline": headline_roofline(),
         "tiles_per_s": round(total / candidates[start_used][2], 1),
         "tiles_per_s_nominal_16x16": round(len(frames_np) * (H // 16) * (W // 16)
                                            / candidates[start_used][2], 1),

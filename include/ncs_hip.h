@@ -54,6 +54,8 @@ extern "C" {
 #define NCS_START_PLAIN 0   /* u0 = data (ncs.c:549-551, ncs.cl:906) */
 #define NCS_START_TILES 1   /* u0 from a caller tile buffer [tiles][256] */
 #define NCS_START_LEARNED 2 /* u0 from the context's conv net */
+#define NCS_START_LINEAR 3  /* u0 from the context's learned linear start,
+                               computed inside the solve */
 
 typedef struct {
   int max_iterations;  /* L-BFGS iteration cap (ncs.cl:61 MAXITERS = 200) */
@@ -202,6 +204,22 @@ int ncs_hip_net_backward(ncs_ctx *ctx, const float *params, const float *x,
                          const void *acts, const float *grad_out,
                          float *grad_params, float *grad_x, int n,
                          void *stream);
+
+/* ---- learned linear start (no reference counterpart) ----
+ *
+ * A linear predictor of the NCS solution in the orthonormal real-DFT basis
+ * R of the 16x16 tile (rows cos k, k = 0..8, then sin k, k = 1..7, of length
+ * 16; Z = R X R^T): with mu = mean d, sigma = sqrt(max(mu, 1) + mean gamma),
+ * x0 = (d - mu) / sigma and x1 = gamma / sigma^2,
+ *   Z(u0) = Z(d) + sigma (A * Z(x0) + B * Z(x1) + c * Z(1)),
+ * coefficient-wise.  It is fitted in closed form on (tile, solution) pairs
+ * the solver produced (ncs_amd.linstart) and evaluated inside the solve
+ * kernel at the cost of three basis changes; the plain start is kept per
+ * tile whenever it scores the lower cost (params->fallback). */
+#define NCS_LINEAR_START_PARAMS 513
+
+/* coef: A[16][16], then B[16][16] (indices [a][b] of Z), then c. */
+int ncs_hip_set_linear_start(ncs_ctx *ctx, const float *coef);
 
 /* ---- autotuner (no reference counterpart) ---- */
 

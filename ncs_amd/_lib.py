@@ -28,7 +28,9 @@ LS_ARMIJO = 1
 START_PLAIN = 0
 START_TILES = 1
 START_LEARNED = 2
+START_LINEAR = 3
 NET_PARAMS = 19393
+LINEAR_START_PARAMS = 513
 INPUT_PHOTONS_F32 = 0
 INPUT_ADU_U16 = 1
 
@@ -116,6 +118,7 @@ SIGNATURES = {
                                         _vp]),
     "ncs_hip_eval_tiles": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _i, _f]),
     "ncs_hip_net_set_weights": (_i, [_vp, _vp]),
+    "ncs_hip_set_linear_start": (_i, [_vp, _vp]),
     "ncs_hip_net_predict_frames_device": (_i, [_vp, _F, _i, _vp, _vp]),
     "ncs_hip_net_forward": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _vp]),
     "ncs_hip_net_backward": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp]),
@@ -146,6 +149,14 @@ def load():
             raise NCSHipError(
                 "libncs_hip.so is not built (expected at %s); run "
                 "`python -m ncs_amd.build` or __graft_entry__.build()" % LIB_PATH)
+        # torch ships its own HIP runtime under the same soname
+        # (libamdhip64.so.7); whichever loads first serves the whole process.
+        # Loading torch's first keeps one runtime, the one that owns torch's
+        # tensors and streams; the other order leaves torch without a device.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         lib = ctypes.cdll.LoadLibrary(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(lib, name)
@@ -236,6 +247,14 @@ class Context(object):
             raise ValueError("the OTF mask must be 16 x 16")
         check(self.lib.ncs_hip_set_otf_mask(self.handle, _ptr(m)), "ncs_hip_set_otf_mask")
         self.mask_fft = m.copy()
+
+    def set_linear_start(self, coef):
+        """Coefficients [A (16x16), B (16x16), c] of the learned linear start."""
+        c = numpy.ascontiguousarray(coef, dtype=numpy.float32)
+        if c.shape != (LINEAR_START_PARAMS,):
+            raise ValueError("expected %d coefficients" % LINEAR_START_PARAMS)
+        check(self.lib.ncs_hip_set_linear_start(self.handle, _ptr(c)),
+              "ncs_hip_set_linear_start")
 
     def reduce_frames(self, datas, gammas, outs, alphas, geometry, params):
         """Host-memory batch; returns per-tile (iterations, status, cost)."""

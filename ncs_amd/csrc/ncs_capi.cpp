@@ -97,6 +97,10 @@ struct ncs_ctx {
   bool has_mask = false;
   bool general = false;
   float mask_fft[256];
+  // host copy of the per-lane tables (mask part and linear-start part)
+  std::vector<float> tab_host = std::vector<float>(kTableFloats, 0.0f);
+  bool has_linear = false;
+  float lin_c = 0.0f;
   // autotuned schedules, keyed by (general mask, padded geometry)
   std::map<std::pair<bool, bool>, Schedule> tuned;
   // last uploaded frame descriptors (pinned staging + completion event)
@@ -207,20 +211,20 @@ extern "C" int ncs_hip_set_otf_mask(ncs_ctx *ctx, const float *mask) {
       qcs[k][l] = 0.5 * (q[k][l] + q[(16 - k) & 15][(16 - l) & 15]);
       qmax = std::max(qmax, std::fabs(qcs[k][l]));
     }
-  std::vector<float> tab(kTableFloats);
+  std::vector<float> &tab = ctx->tab_host;
   double qo_max = 0.0;
   for (int lane = 0; lane < 64; lane++) {
     int g = lane >> 4, x = lane & 15;
     for (int r = 0; r < 4; r++) {
       int b = 4 * g + r, a = x;
-      tab[lane * 16 + r] = (float)rbasis(x, 4 * g + r);
-      tab[lane * 16 + 4 + r] = (float)rbasis(4 * g + r, x);
+      tab[lane * kTableLaneFloats + r] = (float)rbasis(x, 4 * g + r);
+      tab[lane * kTableLaneFloats + 4 + r] = (float)rbasis(4 * g + r, x);
       int k = freq_of(a), l = freq_of(b);
       double qee = 0.5 * (qcs[k][l] + qcs[(16 - k) & 15][l]);
       double qoo = 0.5 * (qcs[k][l] - qcs[(16 - k) & 15][l]);
       double sgn = ((a <= 8) == (b <= 8)) ? -1.0 : 1.0;
-      tab[lane * 16 + 8 + r] = (float)(2.0 * qee);
-      tab[lane * 16 + 12 + r] = (float)(sgn * 2.0 * qoo);
+      tab[lane * kTableLaneFloats + 8 + r] = (float)(2.0 * qee);
+      tab[lane * kTableLaneFloats + 12 + r] = (float)(sgn * 2.0 * qoo);
       qo_max = std::max(qo_max, std::fabs(qoo));
     }
   }
@@ -231,6 +235,29 @@ extern "C" int ncs_hip_set_otf_mask(ncs_ctx *ctx, const float *mask) {
   ctx->general = qo_max > 1e-12 * std::max(qmax, 1e-300);
   std::memcpy(ctx->mask_fft, mask, sizeof(ctx->mask_fft));
   ctx->has_mask = true;
+  return NCS_OK;
+}
+
+extern "C" int ncs_hip_set_linear_start(ncs_ctx *ctx, const float *coef) {
+  if (!ctx || !coef) return fail(NCS_E_ARG, "bad arguments");
+  for (int i = 0; i < NCS_LINEAR_START_PARAMS; i++)
+    if (!std::isfinite(coef[i])) return fail(NCS_E_ARG, "non-finite coefficient");
+  // lane (g, j) slot r holds Z^T[4g + r][j] = Z[j][4g + r]
+  std::vector<float> &tab = ctx->tab_host;
+  for (int lane = 0; lane < 64; lane++) {
+    const int g = lane >> 4, j = lane & 15;
+    for (int r = 0; r < 4; r++) {
+      const int k = j * 16 + 4 * g + r;
+      tab[lane * kTableLaneFloats + 16 + r] = coef[k];
+      tab[lane * kTableLaneFloats + 20 + r] = coef[256 + k];
+    }
+  }
+  HIPCHK(set_device(ctx));
+  HIPCHK(ctx->tables.ensure(tab.size() * sizeof(float)));
+  HIPCHK(hipMemcpy(ctx->tables.p, tab.data(), tab.size() * sizeof(float),
+                   hipMemcpyHostToDevice));
+  ctx->lin_c = coef[512];
+  ctx->has_linear = true;
   return NCS_OK;
 }
 
@@ -277,6 +304,8 @@ static int check_params(const ncs_params *p) {
   if (p->precond != 0 && p->precond != 1) return fail(NCS_E_ARG, "precond must be 0 or 1");
   if (p->waves_per_group < 0 || p->waves_per_group > kMaxWavesPerGroup)
     return fail(NCS_E_ARG, "waves_per_group must be 0..4");
+  if (p->start < NCS_START_PLAIN || p->start > NCS_START_LINEAR)
+    return fail(NCS_E_ARG, "unknown start");
   return NCS_OK;
 }
 
@@ -402,6 +431,8 @@ static SolveArgs make_args(const ncs_ctx *ctx, const FrameDev *d_frames, int n_f
   a.stepm = p->stepm;
   a.fallback = p->fallback;
   a.precond = p->precond;
+  a.lin_start = p->start == NCS_START_LINEAR && u0 == nullptr;
+  a.lin_c = ctx->lin_c;
   return a;
 }
 
@@ -441,6 +472,8 @@ extern "C" int ncs_hip_reduce_frames_device(ncs_ctx *ctx, const ncs_frame *frame
     return fail(NCS_E_ARG, "NCS_START_TILES needs u0_tiles");
   if (params->start == NCS_START_LEARNED && !ctx->has_net)
     return fail(NCS_E_NOWEIGHTS, "learned start requested but no network weights are set");
+  if (params->start == NCS_START_LINEAR && !ctx->has_linear)
+    return fail(NCS_E_NOWEIGHTS, "linear start requested but no coefficients are set");
   std::vector<FrameDev> fd;
   long long total = 0;
   rc = build_frames(frames, n_frames, fd, total);
@@ -481,7 +514,8 @@ extern "C" int ncs_hip_solve_tiles_device(ncs_ctx *ctx, const float *data_in,
   f.alpha = alpha;
   f.geometry = NCS_GEOM_BLOCKS;
   ncs_params p2 = *params;
-  p2.start = u0 ? NCS_START_TILES : NCS_START_PLAIN;
+  if (u0) p2.start = NCS_START_TILES;
+  else if (p2.start != NCS_START_LINEAR) p2.start = NCS_START_PLAIN;
   return ncs_hip_reduce_frames_device(ctx, &f, 1, &p2, u0, iterations, status, cost,
                                       stream);
 }
@@ -496,8 +530,10 @@ extern "C" int ncs_hip_reduce_frames(ncs_ctx *ctx, const ncs_frame *frames, int 
   if (!ctx || !frames || n_frames <= 0) return fail(NCS_E_ARG, "bad arguments");
   int rc = check_params(params);
   if (rc) return rc;
-  if (params->start != NCS_START_PLAIN)
-    return fail(NCS_E_ARG, "host-memory batches use the plain start");
+  if (params->start != NCS_START_PLAIN && params->start != NCS_START_LINEAR)
+    return fail(NCS_E_ARG, "host-memory batches use the plain or the linear start");
+  if (params->start == NCS_START_LINEAR && !ctx->has_linear)
+    return fail(NCS_E_NOWEIGHTS, "linear start requested but no coefficients are set");
   std::vector<FrameDev> fd;
   long long total = 0;
   rc = build_frames(frames, n_frames, fd, total);
@@ -751,7 +787,10 @@ extern "C" int ncs_hip_autotune(ncs_ctx *ctx, const ncs_frame *frames, int n_fra
   int rc = check_params(base);
   if (rc) return rc;
   if (!ctx->has_mask) return fail(NCS_E_NOMASK, "no OTF mask set");
-  if (base->start != NCS_START_PLAIN) return fail(NCS_E_ARG, "the autotuner times the plain start");
+  if (base->start != NCS_START_PLAIN && base->start != NCS_START_LINEAR)
+    return fail(NCS_E_ARG, "the autotuner times the plain or the linear start");
+  if (base->start == NCS_START_LINEAR && !ctx->has_linear)
+    return fail(NCS_E_NOWEIGHTS, "linear start requested but no coefficients are set");
   std::vector<FrameDev> fd;
   long long total = 0;
   rc = build_frames(frames, n_frames, fd, total);
@@ -911,6 +950,8 @@ extern "C" int ncs_hip_pipeline_create(ncs_ctx *ctx, const ncs_pipeline_frame *f
     return fail(NCS_E_ARG, "a pipeline takes the plain or the learned start");
   if (params->start == NCS_START_LEARNED && !ctx->has_net)
     return fail(NCS_E_NOWEIGHTS, "learned start requested but no network weights are set");
+  if (params->start == NCS_START_LINEAR && !ctx->has_linear)
+    return fail(NCS_E_NOWEIGHTS, "linear start requested but no coefficients are set");
   if (input_format != NCS_INPUT_PHOTONS_F32 && input_format != NCS_INPUT_ADU_U16)
     return fail(NCS_E_ARG, "unknown input format");
   if (!ctx->has_mask) return fail(NCS_E_NOMASK, "no OTF mask set");

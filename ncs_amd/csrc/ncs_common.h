@@ -25,8 +25,12 @@ constexpr int kChunk = 4;
 constexpr size_t kQueueBytes = kQueues * kQueueStride * sizeof(unsigned int);
 constexpr float kFitMin = 1.0e-6f; // negative-value guard (ncs.cl:45, ncs.c:169)
 
-// Per-lane OTF tables: 16 floats per lane (see ncs_solver.hip, MaskTables).
-constexpr int kTableFloats = kWave * 16;
+// Per-lane tables, kTableF4 float4 per lane: the real-DFT basis entries
+// (two), the mask's even-even and odd-odd weights, and the learned linear
+// start's two coefficient sets (see ncs_solver.hip, LdsTables).
+constexpr int kTableF4 = 6;
+constexpr int kTableLaneFloats = 4 * kTableF4;
+constexpr int kTableFloats = kWave * kTableLaneFloats;
 
 // One frame of a batch, as the kernel sees it.
 struct FrameDev {
@@ -62,6 +66,8 @@ struct SolveArgs {
   float ftol, gtol, xtol, min_step, max_step, stepm;
   int fallback;
   int precond;                 // spectral initial inverse Hessian (else s.y / y.y)
+  int lin_start;               // learned linear start (coefficients in the tables)
+  float lin_c;                 // its DC bias
 };
 
 }  // namespace ncs

@@ -104,14 +104,22 @@ struct MaskTables {  // held in VGPRs
   __device__ f4 qo() const { return qo_; }
 };
 
+// Lane 0's value (the DC coefficient of a spectral tile sits there).
+__device__ __forceinline__ float lane0(float x) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+}
+
 // The same tables read from LDS at each use: the solver keeps one copy per
 // workgroup there, which saves 12-16 VGPRs per lane across its main loop.
+// Two more entries hold the learned linear start's coefficients.
 struct LdsTables {
-  const f4 *p;  // this lane's 4 entries
+  const f4 *p;  // this lane's kTableF4 entries
   __device__ f4 ka() const { return p[0]; }
   __device__ f4 kb() const { return p[1]; }
   __device__ f4 qe() const { return p[2]; }
   __device__ f4 qo() const { return p[3]; }
+  __device__ f4 la() const { return p[4]; }
+  __device__ f4 lb() const { return p[5]; }
 };
 
 // Real-DFT coefficients of a tile: Z^T = (R X R^T)^T in the MFMA
@@ -508,13 +516,13 @@ void solve_kernel(SolveArgs A) {
   // OTF / DFT tables: each wave writes its lanes' entries (the same values
   // for every wave of the group) and each lane reads back only its own, so
   // no barrier is needed.
-  __shared__ f4 tab_lds[kWave * 4];
+  __shared__ f4 tab_lds[kWave * kTableF4];
   {
-    const f4 *tb = reinterpret_cast<const f4 *>(A.tables + lane * 16);
+    const f4 *tb = reinterpret_cast<const f4 *>(A.tables + lane * 4 * kTableF4);
 #pragma unroll
-    for (int i = 0; i < 4; i++) tab_lds[lane * 4 + i] = tb[i];
+    for (int i = 0; i < kTableF4; i++) tab_lds[lane * kTableF4 + i] = tb[i];
   }
-  const LdsTables T{tab_lds + lane * 4};
+  const LdsTables T{tab_lds + lane * kTableF4};
 
   // Tile schedule.  Device-scope atomics are slow on this part (they are
   // resolved beyond the XCD's L2), so most tiles are assigned statically: wave
@@ -640,15 +648,44 @@ void solve_kernel(SolveArgs A) {
       float gll[4];
       to_spec(u, T, zs);
       nc_spec<GEN>(zs, T, gncs);
-      if (A.u0 == nullptr) {
+      const bool warm = A.u0 != nullptr || A.lin_start;
+      if (!warm) {
         ll_grad_lane(u, gm, c, lin, gll, iv);
       } else {
         // warm start: keep it unless the plain start scores a lower cost
         float both[4];
         both[0] = ll_lane(u, gm, c, lin, gll, iv);
         both[1] = zs[0] * gncs[0] + zs[1] * gncs[1] + (zs[2] * gncs[2] + zs[3] * gncs[3]);
-        float w[4] = {w0[0], w0[1], w0[2], w0[3]}, wz[4], wgs[4], wll[4], wiv[4];
-        to_spec(w, T, wz);
+        float w[4], wz[4], wgs[4], wll[4], wiv[4];
+        if (A.u0 != nullptr) {
+#pragma unroll
+          for (int r = 0; r < 4; r++) w[r] = w0[r];
+          to_spec(w, T, wz);
+        } else {
+          // Learned linear start in the real-DFT basis:
+          //   Z_w = Z_d + sigma (la * Z_x0 + lb * Z_x1 + c e_DC),
+          //   x0 = (d - mu) / sigma, x1 = gamma / sigma^2,
+          //   mu = mean d, sigma = sqrt(max(mu, 1) + mean gamma),
+          // with the tile means read off the DC coefficients (sum / 16).
+          float zg[4], pz[4], wp[4];
+          to_spec(gm, T, zg);
+          const float mu = lane0(zs[0]) * (1.0f / 16.0f);
+          const float mgam = lane0(zg[0]) * (1.0f / 16.0f);
+          const float sig = sqrtf(fmaxf(mu, 1.0f) + mgam);
+          const float isig = rcpf(sig);
+          const f4 la = T.la(), lb = T.lb();
+#pragma unroll
+          for (int r = 0; r < 4; r++)
+            pz[r] = sig * fmaf(la[r], zs[r] * isig, lb[r] * zg[r] * (isig * isig));
+          if (lane == 0)  // DC: x0 has none; the bias term lives here
+            pz[0] = sig * fmaf(lb[0], zg[0] * (isig * isig), 16.0f * A.lin_c);
+          to_pix(pz, T, wp);
+#pragma unroll
+          for (int r = 0; r < 4; r++) {
+            w[r] = u[r] + wp[r];
+            wz[r] = zs[r] + pz[r];
+          }
+        }
         nc_spec<GEN>(wz, T, wgs);
         both[2] = ll_lane(w, gm, c, lin, wll, wiv);
         both[3] = wz[0] * wgs[0] + wz[1] * wgs[1] + (wz[2] * wgs[2] + wz[3] * wgs[3]);
@@ -985,7 +1022,7 @@ __global__ __launch_bounds__(256) void eval_kernel(const float *data, const floa
   if (t >= n) return;  // whole waves leave together
   MaskTables T;
   {
-    const f4 *tb = reinterpret_cast<const f4 *>(tables + lane * 16);
+    const f4 *tb = reinterpret_cast<const f4 *>(tables + lane * kTableLaneFloats);
     T.ka_ = tb[0];
     T.kb_ = tb[1];
     T.qe_ = tb[2];
