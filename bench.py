@@ -72,20 +72,20 @@ def _make_frames(n, pool_size):
     return [_frame(s) for s in range(1, n + 1)]
 
 
-def profiled(key, kernel):
-    """A per-launch figure of `kernel` on this workload from the committed
-    rocprofv3 summaries, or None: "traffic_bytes_per_launch" (FETCH_SIZE and
-    WRITE_SIZE passes calibrated on a device copy) or
-    "mfma_instructions_per_launch" (SQ_INSTS_MFMA)."""
+def profiled(key, kernel, path):
+    """A per-launch figure of `kernel` run on `path` (plain, tuned, linear)
+    for this workload from the committed rocprofv3 summaries, or None:
+    "traffic_bytes_per_launch" (FETCH_SIZE and WRITE_SIZE passes calibrated
+    on a device copy) or "mfma_instructions_per_launch" (SQ_INSTS_MFMA)."""
     import re
     d = os.path.join(ROOT, "profiles")
-    pat = re.compile(r"^%s (\S+): (\d+)$" % re.escape(key))
+    pat = re.compile(r"^%s (\S+) (\S+): (\d+)$" % re.escape(key))
     for fn in (sorted(os.listdir(d)) if os.path.isdir(d) else []):
         with open(os.path.join(d, fn)) as fh:
             for line in fh:
                 m = pat.match(line.strip())
-                if m and m.group(1) == kernel:
-                    return int(m.group(2))
+                if m and m.group(1) == kernel and m.group(2) == path:
+                    return int(m.group(3))
     return None
 
 
@@ -133,9 +133,13 @@ def run_gpu(frames_np, args, weights, lin_coef=None):
     results = {}
 
     def timed(params, label, steps, warmup):
+        # The timed launches return the image, iterations and status, as
+        # the reference's reduceNoise returns the image; the per-tile costs
+        # the parity checks use come from one more launch after the timing
+        # (a tile's result does not depend on the launch).
         for _ in range(warmup):
             ctx.reduce_frames_device(desc, params, None, iters.data_ptr(), status.data_ptr(),
-                                     cost.data_ptr(), stream.cuda_stream)
+                                     None, stream.cuda_stream)
         torch.cuda.synchronize()
         ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
               for _ in range(steps)]
@@ -143,11 +147,14 @@ def run_gpu(frames_np, args, weights, lin_coef=None):
         for e0, e1 in ev:
             e0.record(stream)
             ctx.reduce_frames_device(desc, params, None, iters.data_ptr(), status.data_ptr(),
-                                     cost.data_ptr(), stream.cuda_stream)
+                                     None, stream.cuda_stream)
             e1.record(stream)
         torch.cuda.synchronize()
         wall = (time.perf_counter() - t0) / steps
         kms = [e0.elapsed_time(e1) for e0, e1 in ev]
+        ctx.reduce_frames_device(desc, params, None, iters.data_ptr(), status.data_ptr(),
+                                 cost.data_ptr(), stream.cuda_stream)
+        torch.cuda.synchronize()
         it = iters.cpu().numpy()
         st = status.cpu().numpy()
         results[label] = dict(wall_s=wall, kernel_ms=float(numpy.mean(kms)),
@@ -630,7 +637,8 @@ def main():
     ap.add_argument("--frames", type=int, default=N_FRAMES)
     ap.add_argument("--quick", action="store_true", help="skip the fixed-iteration run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--no-learned", action="store_true", help="skip the learned warm start")
+    ap.add_argument("--no-learned", action="store_true", help="skip the conv-net warm start")
+    ap.add_argument("--no-linear", action="store_true", help="skip the learned linear start")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a 1 GiB device copy first (PMC byte-counter calibration)")
@@ -658,7 +666,7 @@ def main():
             if os.path.exists(warmstart.DEFAULT_WEIGHTS):
                 weights = warmstart.load()
         lin_coef = None
-        if not args.no_learned:
+        if not args.no_linear:
             from ncs_amd import linstart
             if os.path.exists(linstart.DEFAULT_COEF):
                 lin_coef = linstart.load()
@@ -700,7 +708,7 @@ def main():
         weights = p.cpu().numpy()
         weights_note = "trained in this run (50k pairs, 1500 steps): no stored weights found"
     lin_coef, lin_note = None, None
-    if not args.no_learned:
+    if not args.no_linear:
         from ncs_amd import linstart
         if os.path.exists(linstart.DEFAULT_COEF):
             lin_coef = linstart.load()
@@ -713,7 +721,7 @@ def main():
     alg_bytes = pix * 12.0  # data + gamma in, image out (float32)
     it = plain["iters"].astype(numpy.int64)
 
-    def roofline_of(run, m, warm=False):
+    def roofline_of(run, m, path, warm=False):
         kms = run["kernel_ms"]
         it = run["iters"].astype(numpy.float64)
         # trials T = I: one trial per iteration, a lower bound (the first
@@ -724,16 +732,16 @@ def main():
         return {"bound": "mfma", "achieved": round(achieved, 3),
                 "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
-                "traffic": profiled("traffic_bytes_per_launch", kernel),
+                "traffic": profiled("traffic_bytes_per_launch", kernel, path),
                 "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0, "
                                 "profiles/)",
                 "flop_model": "algorithmic: F_eval (1 + T) + 512 T + (13 + 8m) 256 I per tile, "
                               "F_eval = 102,656 (DFT-matrix evaluation), T = I",
                 "kernel": kernel, "kernel_ms": round(kms, 3),
                 "executed_mfma_tflops": (
-                    round(profiled("mfma_instructions_per_launch", kernel) * 2048.0
+                    round(profiled("mfma_instructions_per_launch", kernel, path) * 2048.0
                           / (kms * 1e-3) / 1e12, 2)
-                    if profiled("mfma_instructions_per_launch", kernel) else None),
+                    if profiled("mfma_instructions_per_launch", kernel, path) else None),
                 "flop_per_launch": flops,
                 "hbm_gbs_algorithmic": round(alg_bytes / (kms * 1e-3) / 1e9, 2)}
 
@@ -804,10 +812,10 @@ def main():
 
     def headline_roofline():
         if start_used in ("tuned", "linear"):
-            return roofline_of(res[start_used], res["tune_history"]["history"],
+            return roofline_of(res[start_used], res["tune_history"]["history"], start_used,
                                warm=start_used == "linear")
         # the learned conv-net path runs two kernels; its solve is reported by the plain one
-        return roofline_of(plain, HISTORY)
+        return roofline_of(plain, HISTORY, "plain")
     line = {
         "metric": METRIC,
         "value": round(value, 2),

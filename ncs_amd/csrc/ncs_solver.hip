@@ -44,29 +44,38 @@ __device__ __forceinline__ float dppmov(float v) {
       __builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, RM, BM, false));
 }
 
-__device__ __forceinline__ float lane63(float x) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 63));
+// Sum of N values over the 64 lanes; every lane receives the totals.
+// Four values share one register: two levels of gfx950 row swaps
+// (permlane16_swap exchanges odd rows of one register with even rows of
+// another, permlane32_swap the upper half of one with the lower half of the
+// other) and adds leave row k holding value k's lane-wise sums over the four
+// rows; four DPP butterflies within the row (quad_perm, half-mirror, mirror)
+// finish every row at once, and row k's total is read from lane 16 k.
+__device__ __forceinline__ void swap_add16(float a, float b, float &s) {
+  const auto p = __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b),
+                                                  false, false);
+  s = __uint_as_float(p[0]) + __uint_as_float(p[1]);
 }
 
-// Sum of N values over the 64 lanes; every lane receives the totals.
-// Row sums by DPP butterflies (quad_perm, half-mirror, mirror), rows combined
-// with row_bcast15 / row_bcast31 into lane 63, then broadcast by readlane.
 template <int N>
 __device__ __forceinline__ void wave_sum(float (&x)[N]) {
 #pragma unroll
-  for (int i = 0; i < N; i++) x[i] += dppmov<0xB1>(x[i]);
+  for (int g0 = 0; g0 < N; g0 += 4) {
+    const int n = N - g0 < 4 ? N - g0 : 4;
+    float s01, s23 = 0.0f;
+    swap_add16(x[g0], n > 1 ? x[g0 + 1] : 0.0f, s01);  // rows [a01, b01, a23, b23]
+    if (n > 2) swap_add16(x[g0 + 2], n > 3 ? x[g0 + 3] : 0.0f, s23);
+    const auto q = __builtin_amdgcn_permlane32_swap(__float_as_uint(s01), __float_as_uint(s23),
+                                                    false, false);
+    float s = __uint_as_float(q[0]) + __uint_as_float(q[1]);  // row k: value k
+    s += dppmov<0xB1>(s);
+    s += dppmov<0x4E>(s);
+    s += dppmov<0x141>(s);
+    s += dppmov<0x140>(s);
 #pragma unroll
-  for (int i = 0; i < N; i++) x[i] += dppmov<0x4E>(x[i]);
-#pragma unroll
-  for (int i = 0; i < N; i++) x[i] += dppmov<0x141>(x[i]);
-#pragma unroll
-  for (int i = 0; i < N; i++) x[i] += dppmov<0x140>(x[i]);
-#pragma unroll
-  for (int i = 0; i < N; i++) x[i] += dppmov<0x142, 0xA>(x[i]);
-#pragma unroll
-  for (int i = 0; i < N; i++) x[i] += dppmov<0x143, 0xC>(x[i]);
-#pragma unroll
-  for (int i = 0; i < N; i++) x[i] = lane63(x[i]);
+    for (int k = 0; k < n; k++)
+      x[g0 + k] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(s), 16 * k));
+  }
 }
 
 __device__ __forceinline__ float wave_dot(const float (&a)[4], const float (&b)[4]) {
@@ -77,16 +86,22 @@ __device__ __forceinline__ float wave_dot(const float (&a)[4], const float (&b)[
 
 __device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
 
+// Natural log of a normal, positive float on the transcendental unit:
+// v_log_f32 (log2) times ln 2, two instructions.  The libm-accurate logf
+// adds denormal rescaling and an extended-precision ln 2 product (about a
+// dozen instructions) that the solver's arguments (>= 1e-6) never need.
+__device__ __forceinline__ float ln_hw(float x) {
+  return __builtin_amdgcn_logf(x) * 0.693147182f;
+}
+
 // log(1 + x) for x > -1, branch-free: the rounding error of u = 1 + x is
 // added back to log(u), log1p(x) = log(u) + (x - (u - 1)) / u (the
 // correction term of Goldberg's log1p), with 1/u taken as 2 - u: the
 // correction is below half an ulp of 1, so the x^2 error of that factor is
-// negligible.  With the transcendental unit's log2, measured on gfx950 over
-// |x| in [1e-8, 0.25] (with 1/u exact): max relative error 2.3e-7, as
-// accurate as a degree-9 atanh series.
+// negligible.
 __device__ __forceinline__ float log1p_acc(float x) {
   const float u = 1.0f + x;
-  return fmaf(x - (u - 1.0f), 2.0f - u, __logf(u));
+  return fmaf(x - (u - 1.0f), 2.0f - u, ln_hw(u));
 }
 
 // ---------------------------------------------------------------------------
@@ -423,7 +438,7 @@ __device__ __forceinline__ float ll_value_lane(const float (&u)[4], const float 
 #pragma unroll
   for (int r = 0; r < 4; r++) {
     const float v = fmaxf(u[r] + gm[r], kFitMin);
-    s += lin[r] * u[r] - c[r] * __logf(v);
+    s += lin[r] * u[r] - c[r] * ln_hw(v);
   }
   return s;
 }
