@@ -166,14 +166,14 @@ def run_gpu(frames_np, args, weights, lin_coef=None):
                              dict(zip(*[x.tolist() for x in numpy.unique(st, return_counts=True)]))),
               file=sys.stderr, flush=True)
 
-    # schedule: grid and workgroup size tuned on the first frame (history
+    # schedule: grid and workgroup size tuned on the whole batch (history
     # stays m = 5, the workload's)
     sched = None
     if args.schedule_cache and os.path.exists(args.schedule_cache):
         with open(args.schedule_cache) as fh:
             sched = json.load(fh)
     if sched is None:
-        tune = ctx.autotune(desc[:1], _lib.default_params(history=HISTORY, max_iterations=MAX_IT),
+        tune = ctx.autotune(desc, _lib.default_params(history=HISTORY, max_iterations=MAX_IT),
                             tune_history=False)
     else:
         tune = sched["m5"]
@@ -188,11 +188,8 @@ def run_gpu(frames_np, args, weights, lin_coef=None):
     timed(scalar, "plain_scalar_h0", max(1, min(args.steps, 3)), 1)
     # the autotuner also picks the history depth (history=0 below)
     if sched is None:
-        tune_h = ctx.autotune(desc[:1], _lib.default_params(max_iterations=MAX_IT),
+        tune_h = ctx.autotune(desc, _lib.default_params(max_iterations=MAX_IT),
                               tune_history=True)
-        if args.schedule_cache:
-            with open(args.schedule_cache, "w") as fh:
-                json.dump({"m5": tune, "tuned": tune_h}, fh)
     else:
         tune_h = sched["tuned"]
     print("[bench] autotune with depth: %s" % tune_h, file=sys.stderr, flush=True)
@@ -201,14 +198,27 @@ def run_gpu(frames_np, args, weights, lin_coef=None):
                               waves_per_cu=tune_h["waves_per_cu"],
                               waves_per_group=tune_h["waves_per_group"]),
           "tuned", args.steps, args.warmup)
+    tune_l = None
     if lin_coef is not None:
-        # learned linear start, fused into the solve kernel
+        # learned linear start, fused into the solve kernel, with a schedule
+        # tuned for it (its tiles take fewer iterations)
         ctx.set_linear_start(lin_coef)
-        timed(_lib.default_params(history=tune_h["history"], max_iterations=MAX_IT,
-                                  waves_per_cu=tune_h["waves_per_cu"],
-                                  waves_per_group=tune_h["waves_per_group"],
+        if sched is None or "linear" not in sched:
+            tune_l = ctx.autotune(desc, _lib.default_params(max_iterations=MAX_IT,
+                                                            start=_lib.START_LINEAR),
+                                  tune_history=True)
+        else:
+            tune_l = sched["linear"]
+        print("[bench] autotune with the linear start: %s" % tune_l, file=sys.stderr, flush=True)
+        results["tune_linear"] = tune_l
+        timed(_lib.default_params(history=tune_l["history"], max_iterations=MAX_IT,
+                                  waves_per_cu=tune_l["waves_per_cu"],
+                                  waves_per_group=tune_l["waves_per_group"],
                                   start=_lib.START_LINEAR),
               "linear", args.steps, args.warmup)
+    if sched is None and args.schedule_cache:
+        with open(args.schedule_cache, "w") as fh:
+            json.dump({"m5": tune, "tuned": tune_h, "linear": tune_l}, fh)
     if weights is not None:
         from ncs_amd import warmstart
         warmstart.set_context_weights(weights)
@@ -768,13 +778,14 @@ def main():
         L = res["linear"]
         lm = pix / L["wall_s"] / 1e6
         rel = numpy.abs(L["cost"].astype(numpy.float64) - plain["cost"]) / numpy.abs(plain["cost"])
-        ref_it = res["tuned"]["iters"] if "tuned" in res else plain["iters"]
+        TL = res["tune_linear"]
         linear_info = {
             "value": round(lm, 2), "unit": "Mpix/s", "ms_per_step": round(L["wall_s"] * 1e3, 3),
             "kernel_ms": round(L["kernel_ms"], 3),
-            "history": res["tune_history"]["history"],
+            "history": TL["history"], "waves_per_cu": TL["waves_per_cu"],
+            "waves_per_group": TL["waves_per_group"],
             "iterations_mean": round(float(L["iters"].mean()), 3),
-            "iterations_saved_vs_same_history_plain": round(float(ref_it.mean() - L["iters"].mean()), 3),
+            "iterations_saved_vs_plain_m5": round(float(plain["iters"].mean() - L["iters"].mean()), 3),
             "cost_vs_plain_max_rel": float(rel.max()),
             "cost_vs_plain_p99_rel": float(numpy.percentile(rel, 99)),
             "matched_1e5": bool(rel.max() <= 1e-5),
@@ -811,9 +822,10 @@ def main():
             value, start_used = v, name
 
     def headline_roofline():
-        if start_used in ("tuned", "linear"):
-            return roofline_of(res[start_used], res["tune_history"]["history"], start_used,
-                               warm=start_used == "linear")
+        if start_used == "tuned":
+            return roofline_of(res["tuned"], res["tune_history"]["history"], "tuned")
+        if start_used == "linear":
+            return roofline_of(res["linear"], res["tune_linear"]["history"], "linear", warm=True)
         # the learned conv-net path runs two kernels; its solve is reported by the plain one
         return roofline_of(plain, HISTORY, "plain")
     line = {
@@ -843,6 +855,8 @@ def main():
         "plain_scalar_h0": _scalar_summary(res, plain, pix),
         "tuned": tuned_info,
         "autotune": res.get("tune"),
+        "autotune_history": res.get("tune_history"),
+        "autotune_linear": res.get("tune_linear"),
         "linear_start": linear_info,
         "learned_warm": learned_info,
         "roofline": headline_roofline(),

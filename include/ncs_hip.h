@@ -237,7 +237,8 @@ typedef struct {
  * for ncs_hip_reduce_frames_device; nothing is written to their outputs) over
  * candidate schedules -- waves per CU up to the occupancy that the history's
  * LDS rings allow, 1/2/4 tiles per workgroup and, if tune_history is set,
- * history depths 1, 2, 3, 5 and 8 -- and keeps the fastest in the context's
+ * history depths 1, 2, 3, 5 and 8 -- each the best of three timed launches
+ * after an untimed one -- and keeps the fastest in the context's
  * cache for the mask kind and geometry of the sample.  Later calls whose
  * params leave waves_per_cu, waves_per_group or history at 0 use it.
  * Synchronous. */

@@ -820,11 +820,11 @@ extern "C" int ncs_hip_autotune(ncs_ctx *ctx, const ncs_frame *frames, int n_fra
   }
   const SolveArgs a = make_args(ctx, dfr.as<FrameDev>(), n_frames, total, base, nullptr,
                                 nullptr, nullptr, nullptr, nullptr, q.as<unsigned int>());
-  // one untimed run (code load, caches), then the best of two
+  // one untimed run (code load, caches), then the best of three
   auto time_one = [&](const Schedule &sc, float *ms) -> hipError_t {
     hipError_t r = launch_schedule(ctx, sc, pad, total, s, a);
     float best = 3.0e38f;
-    for (int rep = 0; rep < 2 && r == hipSuccess; rep++) {
+    for (int rep = 0; rep < 3 && r == hipSuccess; rep++) {
       r = hipEventRecord(e0, s);
       if (r == hipSuccess) r = launch_schedule(ctx, sc, pad, total, s, a);
       if (r == hipSuccess) r = hipEventRecord(e1, s);

@@ -409,7 +409,9 @@ struct TileGeom {
 
 __device__ __forceinline__ TileGeom tile_geom(const FrameConst &F, long long local) {
   TileGeom G;
-  int a = (int)(local / F.ty), b = (int)(local % F.ty);
+  // tile indices fit 32 bits (the host caps a batch below 2^32 tiles)
+  const unsigned int l32 = (unsigned int)local, ty = (unsigned int)F.ty;
+  const int a = (int)(l32 / ty), b = (int)(l32 - (unsigned int)a * ty);
   if (F.geom == NCS_GEOM_REFERENCE) {
     const int lastx = F.H - 15, lasty = F.W - 15;
     G.bx = min(-1 + 14 * a, lastx);
