@@ -911,7 +911,7 @@ def main():
     if not args.no_cpu:
         cb = cpu_baseline(frames_np, args.cpu_budget)
         line["cpu_baseline"] = cb
-        line["gpu_over_cpu"] = round(mpix / cb["value"], 1)
+        line["gpu_over_cpu"] = round(value / cb["value"], 1)
     print(json.dumps(line), flush=True)
 
 
