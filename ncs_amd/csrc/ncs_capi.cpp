@@ -196,6 +196,14 @@ static double rbasis(int a, int i) {
 
 static int freq_of(int a) { return a <= 8 ? a : a - 8; }
 
+// v = hi + lo in FP16 (round to nearest even), |v - hi - lo| <~ 2^-22 |v|.
+static void split_f16(double v, uint16_t &hi, uint16_t &lo) {
+  const _Float16 h = (_Float16)(float)v;
+  const _Float16 l = (_Float16)(float)(v - (double)(float)h);
+  std::memcpy(&hi, &h, 2);
+  std::memcpy(&lo, &l, 2);
+}
+
 extern "C" int ncs_hip_set_otf_mask(ncs_ctx *ctx, const float *mask) {
   if (!ctx || !mask) return fail(NCS_E_ARG, "bad arguments");
   double q[16][16], qcs[16][16];
@@ -219,6 +227,16 @@ extern "C" int ncs_hip_set_otf_mask(ncs_ctx *ctx, const float *mask) {
       int b = 4 * g + r, a = x;
       tab[lane * kTableLaneFloats + r] = (float)rbasis(x, 4 * g + r);
       tab[lane * kTableLaneFloats + 4 + r] = (float)rbasis(4 * g + r, x);
+      // FP16 hi/lo pairs of the same entries, each duplicated into both
+      // halves of an 8-wide FP16 operand (slots r and 4 + r)
+      for (int t = 0; t < 2; t++) {
+        const double v = t == 0 ? rbasis(x, 4 * g + r) : rbasis(4 * g + r, x);
+        uint16_t hi, lo;
+        split_f16(v, hi, lo);
+        uint16_t *h = reinterpret_cast<uint16_t *>(&tab[lane * kTableLaneFloats + 24 + 8 * t]);
+        h[r] = h[4 + r] = hi;      // entry 6 (t = 0) or 8 (t = 1): hi
+        h[8 + r] = h[12 + r] = lo;  // entry 7 or 9: lo
+      }
       int k = freq_of(a), l = freq_of(b);
       double qee = 0.5 * (qcs[k][l] + qcs[(16 - k) & 15][l]);
       double qoo = 0.5 * (qcs[k][l] - qcs[(16 - k) & 15][l]);

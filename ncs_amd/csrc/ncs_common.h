@@ -25,10 +25,15 @@ constexpr int kChunk = 4;
 constexpr size_t kQueueBytes = kQueues * kQueueStride * sizeof(unsigned int);
 constexpr float kFitMin = 1.0e-6f; // negative-value guard (ncs.cl:45, ncs.c:169)
 
-// Per-lane tables, kTableF4 float4 per lane: the real-DFT basis entries
-// (two), the mask's even-even and odd-odd weights, and the learned linear
-// start's two coefficient sets (see ncs_solver.hip, LdsTables).
-constexpr int kTableF4 = 6;
+// Per-lane tables, kTableF4 16-byte entries per lane: the real-DFT basis
+// entries in FP32 (two), the mask's even-even and odd-odd weights, the
+// learned linear start's two coefficient sets, and the basis entries again
+// as FP16 hi/lo pairs for the FP16 matrix-core basis changes (four; see
+// ncs_solver.hip, LdsTables).  The solver keeps entries kSolveTableFirst..
+// in LDS.
+constexpr int kTableF4 = 10;
+constexpr int kSolveTableFirst = 2;
+constexpr int kSolveTableF4 = kTableF4 - kSolveTableFirst;
 constexpr int kTableLaneFloats = 4 * kTableF4;
 constexpr int kTableFloats = kWave * kTableLaneFloats;
 

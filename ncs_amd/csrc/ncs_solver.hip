@@ -33,6 +33,8 @@
 namespace ncs {
 
 typedef float f4 __attribute__((ext_vector_type(4)));
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u4 __attribute__((ext_vector_type(4)));
 
 // ---------------------------------------------------------------------------
 // Wave primitives
@@ -76,6 +78,36 @@ __device__ __forceinline__ void wave_sum(float (&x)[N]) {
     for (int k = 0; k < n; k++)
       x[g0 + k] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(s), 16 * k));
   }
+}
+
+// Largest of non-negative per-lane values over the wave (uniform): DPP
+// butterflies within rows, then the two row swaps.
+__device__ __forceinline__ float wave_max(float v) {
+  v = fmaxf(v, dppmov<0xB1>(v));
+  v = fmaxf(v, dppmov<0x4E>(v));
+  v = fmaxf(v, dppmov<0x141>(v));
+  v = fmaxf(v, dppmov<0x140>(v));
+  const auto p = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v),
+                                                  false, false);
+  v = fmaxf(__uint_as_float(p[0]), __uint_as_float(p[1]));
+  const auto q = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v),
+                                                  false, false);
+  v = fmaxf(__uint_as_float(q[0]), __uint_as_float(q[1]));
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
+// Exact power-of-two scale for the FP16 basis changes of a tile's
+// data-like operands (iterates, directions, gamma, the learned
+// correction): it brings the tile's largest |data| into [32, 64), which
+// leaves FP16 a factor of about 200 of headroom (an overflow beyond it
+// falls back to FP32), and values down to about 1/400 of the maximum keep
+// a normal FP16 low half.  Zero, subnormal or non-finite maxima give 1.
+__device__ __forceinline__ float tile_scale(const float (&x)[4]) {
+  const float m = wave_max(
+      fmaxf(fmaxf(fabsf(x[0]), fabsf(x[1])), fmaxf(fabsf(x[2]), fabsf(x[3]))));
+  const int e = (int)(__float_as_uint(m) >> 23);  // biased exponent
+  // 2^(132 - e) has biased exponent 259 - e
+  return (e >= 5 && e <= 254) ? __uint_as_float((unsigned int)(259 - e) << 23) : 1.0f;
 }
 
 __device__ __forceinline__ float wave_dot(const float (&a)[4], const float (&b)[4]) {
@@ -124,17 +156,35 @@ __device__ __forceinline__ float lane0(float x) {
   return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
 }
 
-// The same tables read from LDS at each use: the solver keeps one copy per
-// workgroup there, which saves 12-16 VGPRs per lane across its main loop.
-// Two more entries hold the learned linear start's coefficients.
+// The solver's tables, read from LDS at each use (one copy per workgroup,
+// which saves 12-16 VGPRs per lane across the main loop): the mask weights,
+// the learned linear start's coefficients, and the basis entries as FP16
+// hi/lo pairs, each duplicated into slots r and 4 + r of an 8-wide operand.
 struct LdsTables {
-  const f4 *p;  // this lane's kTableF4 entries
-  __device__ f4 ka() const { return p[0]; }
-  __device__ f4 kb() const { return p[1]; }
-  __device__ f4 qe() const { return p[2]; }
-  __device__ f4 qo() const { return p[3]; }
-  __device__ f4 la() const { return p[4]; }
-  __device__ f4 lb() const { return p[5]; }
+  const f4 *p;  // this lane's kSolveTableF4 entries (kSolveTableFirst..)
+  __device__ f4 qe() const { return p[0]; }
+  __device__ f4 qo() const { return p[1]; }
+  __device__ f4 la() const { return p[2]; }
+  __device__ f4 lb() const { return p[3]; }
+  __device__ h8 kah() const { return __builtin_bit_cast(h8, p[4]); }
+  __device__ h8 kal() const { return __builtin_bit_cast(h8, p[5]); }
+  __device__ h8 kbh() const { return __builtin_bit_cast(h8, p[6]); }
+  __device__ h8 kbl() const { return __builtin_bit_cast(h8, p[7]); }
+};
+
+// The FP32 basis entries, read from the global table when the FP32 basis
+// changes run (past the precision switch, or as the FP16 path's fallback).
+// The lane offset is formed where it is used, so no VGPRs stay reserved for
+// it across the solver's loop.
+struct GlobalBasis {
+  const float *tables;  // uniform base of the per-lane table
+  __device__ const f4 *lane_p() const {
+    int ln = __lane_id();
+    asm volatile("" : "+v"(ln));
+    return reinterpret_cast<const f4 *>(tables + ln * kTableLaneFloats);
+  }
+  __device__ f4 ka() const { return lane_p()[0]; }
+  __device__ f4 kb() const { return lane_p()[1]; }
 };
 
 // Real-DFT coefficients of a tile: Z^T = (R X R^T)^T in the MFMA
@@ -177,6 +227,110 @@ __device__ __forceinline__ void to_pix(const float (&w)[4], const TB &T, float (
   g = __builtin_amdgcn_mfma_f32_16x16x4f32(kb[3], e[3], g, 0, 0, 0);
 #pragma unroll
   for (int r = 0; r < 4; r++) x[r] = g[r];
+}
+
+// ---------------------------------------------------------------------------
+// The same basis changes on the FP16 matrix cores.  FP32 MFMA shares the
+// SIMD's vector datapath (it does not overlap VALU work), FP16 MFMA runs on
+// the matrix cores at 8x the work per instruction and partly overlaps it.
+// Each FP32 operand x becomes an FP16 pair hi = rn(x), lo = rn(x - hi)
+// (x - hi is exact), so hi + lo carries 22 significant bits; with the basis
+// split the same way and FP32 accumulation, one 16x16x32 MFMA takes
+// [hi | lo] of 16 K values against one basis half, and two MFMA per GEMM
+// stage give the product to about FP32 accuracy.  The K slot order (slots
+// r and 4 + r of lane group g are tile row 4 g + r) is the same in both
+// operands, which is all the contraction needs.  Operands must lie well
+// inside FP16's range: callers scale by an exact power of two, and a
+// non-finite result (overflow, or a non-finite input) is recomputed in FP32.
+// ---------------------------------------------------------------------------
+
+// Relative gradient norm below which the solver's basis changes switch from
+// the FP16 split (about 2^-22 per operand) to FP32: the default tolerance
+// (1e-5) stops at it, tighter tolerances finish in FP32.
+constexpr float kF16Floor = 1.0e-5f;
+
+// [rn(x0..3), rn(x0..3 - hi)] as one 8-wide FP16 operand: 8 VALU.  The
+// packed conversions are compiler code on purpose: they are the first
+// readers of x, which often is an MFMA result, and the compiler inserts the
+// MFMA-to-VALU wait states only before instructions it can see.  Only the
+// residual x - hi is inline asm (v_fma_mix_f32 takes the packed FP16 hi
+// directly; the compiler would convert it back to FP32 first), and it
+// depends on the conversion, so it always comes after those waits.
+typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ h8 split_f16(float x0, float x1, float x2, float x3) {
+  const h2 h01 = {(_Float16)x0, (_Float16)x1};
+  const h2 h23 = {(_Float16)x2, (_Float16)x3};
+  const unsigned int u01 = __builtin_bit_cast(unsigned int, h01);
+  const unsigned int u23 = __builtin_bit_cast(unsigned int, h23);
+  float r0, r1, r2, r3;
+  asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(r0) : "v"(u01), "v"(x0));
+  asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]"
+      : "=v"(r1) : "v"(u01), "v"(x1));
+  asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(r2) : "v"(u23), "v"(x2));
+  asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]"
+      : "=v"(r3) : "v"(u23), "v"(x3));
+  const h2 l01 = {(_Float16)r0, (_Float16)r1};
+  const h2 l23 = {(_Float16)r2, (_Float16)r3};
+  return (h8){h01[0], h01[1], h23[0], h23[1], l01[0], l01[1], l23[0], l23[1]};
+}
+
+// Every lane's four values finite (uniform).
+__device__ __forceinline__ bool all_finite(const float (&z)[4]) {
+  const bool ok = __builtin_isfinite(z[0]) & __builtin_isfinite(z[1]) &
+                  __builtin_isfinite(z[2]) & __builtin_isfinite(z[3]);
+  return __all(ok);
+}
+
+// to_spec on the FP16 matrix cores: z = to_spec(x * s) / s (s a power of
+// two).  4 MFMA 16x16x32 f16; in FP32 when `exact` (uniform) is set.
+__device__ __forceinline__ void to_spec16(const float (&x)[4], const LdsTables &T,
+                                          const GlobalBasis &G, float s, float (&z)[4],
+                                          bool exact) {
+  if (exact) {
+    to_spec(x, G, z);
+    return;
+  }
+  const h8 th = T.kah(), tl = T.kal();
+  const h8 xs = split_f16(x[0] * s, x[1] * s, x[2] * s, x[3] * s);
+  f4 y = {0.f, 0.f, 0.f, 0.f};
+  // (R X)^T = X^T R^T : data as the A operand
+  y = __builtin_amdgcn_mfma_f32_16x16x32_f16(xs, th, y, 0, 0, 0);
+  y = __builtin_amdgcn_mfma_f32_16x16x32_f16(xs, tl, y, 0, 0, 0);
+  const h8 ys = split_f16(y[0], y[1], y[2], y[3]);
+  f4 zz = {0.f, 0.f, 0.f, 0.f};
+  // Z^T = R (R X)^T : data as the B operand
+  zz = __builtin_amdgcn_mfma_f32_16x16x32_f16(th, ys, zz, 0, 0, 0);
+  zz = __builtin_amdgcn_mfma_f32_16x16x32_f16(tl, ys, zz, 0, 0, 0);
+  const float is = 1.0f / s;
+#pragma unroll
+  for (int r = 0; r < 4; r++) z[r] = zz[r] * is;
+  if (!all_finite(z)) to_spec(x, G, z);
+}
+
+// to_pix on the FP16 matrix cores: x = to_pix(w * s) / s.  4 MFMA; in
+// FP32 when `exact` is set.
+__device__ __forceinline__ void to_pix16(const float (&w)[4], const LdsTables &T,
+                                         const GlobalBasis &G, float s, float (&x)[4],
+                                         bool exact) {
+  if (exact) {
+    to_pix(w, G, x);
+    return;
+  }
+  const h8 th = T.kbh(), tl = T.kbl();
+  const h8 ws = split_f16(w[0] * s, w[1] * s, w[2] * s, w[3] * s);
+  f4 e = {0.f, 0.f, 0.f, 0.f};
+  // W R : W^T held, so data as the A operand
+  e = __builtin_amdgcn_mfma_f32_16x16x32_f16(ws, th, e, 0, 0, 0);
+  e = __builtin_amdgcn_mfma_f32_16x16x32_f16(ws, tl, e, 0, 0, 0);
+  const h8 es = split_f16(e[0], e[1], e[2], e[3]);
+  f4 g = {0.f, 0.f, 0.f, 0.f};
+  // R^T (W R)
+  g = __builtin_amdgcn_mfma_f32_16x16x32_f16(th, es, g, 0, 0, 0);
+  g = __builtin_amdgcn_mfma_f32_16x16x32_f16(tl, es, g, 0, 0, 0);
+  const float is = 1.0f / s;
+#pragma unroll
+  for (int r = 0; r < 4; r++) x[r] = g[r] * is;
+  if (!all_finite(x)) to_pix(w, G, x);
 }
 
 // The NC Hessian K in the spectral basis: the diagonal 2 q_ee plus, for a
@@ -533,13 +687,14 @@ void solve_kernel(SolveArgs A) {
   // OTF / DFT tables: each wave writes its lanes' entries (the same values
   // for every wave of the group) and each lane reads back only its own, so
   // no barrier is needed.
-  __shared__ f4 tab_lds[kWave * kTableF4];
+  __shared__ f4 tab_lds[kWave * kSolveTableF4];
   {
-    const f4 *tb = reinterpret_cast<const f4 *>(A.tables + lane * 4 * kTableF4);
+    const f4 *tb = reinterpret_cast<const f4 *>(A.tables + lane * 4 * kTableF4) + kSolveTableFirst;
 #pragma unroll
-    for (int i = 0; i < kTableF4; i++) tab_lds[lane * kTableF4 + i] = tb[i];
+    for (int i = 0; i < kSolveTableF4; i++) tab_lds[lane * kSolveTableF4 + i] = tb[i];
   }
-  const LdsTables T{tab_lds + lane * kTableF4};
+  const LdsTables T{tab_lds + lane * kSolveTableF4};
+  const GlobalBasis GB{A.tables};
 
   // Tile schedule.  Device-scope atomics are slow on this part (they are
   // resolved beyond the XCD's L2), so most tiles are assigned statically: wave
@@ -653,6 +808,7 @@ void solve_kernel(SolveArgs A) {
       c[r] = lin[r] * (raw_d[r] + gm[r]);
       u[r] = raw_d[r];
     }
+    const float st = tile_scale(raw_d);
 
     // Start.  The L-BFGS state (gradient, NC gradient, direction, history)
     // lives in the orthonormal real-DFT basis, where K and the spectral H0
@@ -661,9 +817,14 @@ void solve_kernel(SolveArgs A) {
     // gradient), 8 MFMA each.  Dot products are basis-independent.
     float gs[4], gncs[4], zs[4], iv[4];  // spectral g, K u; pixel 1/(u+gamma)
     float f = 0.0f;  // f(u) - f(u_start): only differences drive the search
+    // Basis changes run on the FP16 matrix cores (22-bit operands) until the
+    // gradient is within kF16Floor of the stop rule's scale, and in FP32
+    // after that: a default-tolerance solve (eps 1e-5) stops before the
+    // switch, a tighter one finishes with FP32 accuracy.
+    bool exact = false;
     {
       float gll[4];
-      to_spec(u, T, zs);
+      to_spec16(u, T, GB, st, zs, exact);
       nc_spec<GEN>(zs, T, gncs);
       const bool warm = A.u0 != nullptr || A.lin_start;
       if (!warm) {
@@ -677,7 +838,7 @@ void solve_kernel(SolveArgs A) {
         if (A.u0 != nullptr) {
 #pragma unroll
           for (int r = 0; r < 4; r++) w[r] = w0[r];
-          to_spec(w, T, wz);
+          to_spec16(w, T, GB, st, wz, exact);
         } else {
           // Learned linear start in the real-DFT basis:
           //   Z_w = Z_d + sigma (la * Z_x0 + lb * Z_x1 + c e_DC),
@@ -685,7 +846,7 @@ void solve_kernel(SolveArgs A) {
           //   mu = mean d, sigma = sqrt(max(mu, 1) + mean gamma),
           // with the tile means read off the DC coefficients (sum / 16).
           float zg[4], pz[4], wp[4];
-          to_spec(gm, T, zg);
+          to_spec16(gm, T, GB, st, zg, exact);
           const float mu = lane0(zs[0]) * (1.0f / 16.0f);
           const float mgam = lane0(zg[0]) * (1.0f / 16.0f);
           const float sig = sqrtf(fmaxf(mu, 1.0f) + mgam);
@@ -696,7 +857,7 @@ void solve_kernel(SolveArgs A) {
             pz[r] = sig * fmaf(la[r], zs[r] * isig, lb[r] * zg[r] * (isig * isig));
           if (lane == 0)  // DC: x0 has none; the bias term lives here
             pz[0] = sig * fmaf(lb[0], zg[0] * (isig * isig), 16.0f * A.lin_c);
-          to_pix(pz, T, wp);
+          to_pix16(pz, T, GB, st, wp, exact);
 #pragma unroll
           for (int r = 0; r < 4; r++) {
             w[r] = u[r] + wp[r];
@@ -720,7 +881,7 @@ void solve_kernel(SolveArgs A) {
         }
       }
       float gls[4];
-      to_spec(gll, T, gls);
+      to_spec16(gll, T, GB, 1.0f, gls, exact);
 #pragma unroll
       for (int r = 0; r < 4; r++) gs[r] = fmaf(alpha, gncs[r], gls[r]);
     }
@@ -739,6 +900,7 @@ void solve_kernel(SolveArgs A) {
     }
     bool done = gg <= A.eps * A.eps * fmaxf(1.0f, uu);
     if (done) status = NCS_SUCCESS;
+    else if (gg <= kF16Floor * kF16Floor * fmaxf(1.0f, uu)) exact = true;
 
     float rho[M], al[M];  // rho[j], al[j]: j-th newest pair
     int nh = 0, head = M - 1;
@@ -787,7 +949,7 @@ void solve_kernel(SolveArgs A) {
         status = NCS_INCREASING_GRADIENT;
         break;
       }
-      to_pix(ds, T, dp);
+      to_pix16(ds, T, GB, st, dp, exact);
 
       // line search; the first trial is taken at the clamped initial step
       float gt[4], ivt[4], phi, dphi_acc = 0.0f;  // dphi_acc: phi'(t) of the accepted trial
@@ -864,11 +1026,11 @@ void solve_kernel(SolveArgs A) {
                                (ut[3] != u[3]));
       gnc_exact = false;
       if ((k & 15) == 0) {  // periodic exact refresh of the NC gradient
-        to_spec(ut, T, zs);
+        to_spec16(ut, T, GB, st, zs, exact);
         nc_spec<GEN>(zs, T, gncn);
         gnc_exact = true;
       }
-      to_spec(gt, T, gts);
+      to_spec16(gt, T, GB, 1.0f, gts, exact);
       f += phi;
       // y.s = t (phi'(t) - phi'(0)) needs no reduction; the third slot is
       // y.y for the scalar initial Hessian, the mean curvature for the
@@ -892,9 +1054,10 @@ void solve_kernel(SolveArgs A) {
         kappa = red[2] * (1.0f / 256.0f);
         yy = red[2];
         done = gg <= A.eps * A.eps * fmaxf(1.0f, uu);
+        if (!done && gg <= kF16Floor * kF16Floor * fmaxf(1.0f, uu)) exact = true;
         if (!done || gnc_exact) break;
         // confirm convergence with an exact NC gradient
-        to_spec(ut, T, zs);
+        to_spec16(ut, T, GB, st, zs, exact);
         nc_spec<GEN>(zs, T, gncn);
         gnc_exact = true;
       }
@@ -985,7 +1148,7 @@ void solve_kernel(SolveArgs A) {
     // final cost at the returned point (when asked for), NC from an exact
     // transform of u
     if (A.cost != nullptr) {
-      to_spec(u, T, zs);
+      to_spec16(u, T, GB, st, zs, exact);
       nc_spec<GEN>(zs, T, gncs);
       float red[2];
       red[0] = ll_value_lane(u, gm, c, lin);
