@@ -1,0 +1,122 @@
+"""GPU checks of the solver's precision machinery.
+
+The basis changes run on the FP16 matrix cores with every operand split
+into an FP16 hi/lo pair.  Data-like operands carry a per-tile power-of-two
+scale, a non-finite result falls back to FP32, and past a 1e-5 relative
+gradient the solver switches to FP32.  Data over nine decades of scale
+must still reach the FP64 oracle's minimiser, to the bars of
+test_gpu_solver.py: final cost within 1e-5 relative, and the FP64 gradient
+within the stop rule.  Overflow is forced in the two transforms that can
+meet it: the learned start's gamma transform and the likelihood-gradient
+transform.
+"""
+import numpy
+import pytest
+
+import oracle
+from ncs_amd import _lib, masks, sim
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _tiles(seed, n):
+    fr = sim.make_frame(seed, (128, 128))
+    rng = numpy.random.default_rng(seed)
+    ys = rng.integers(0, 112, n)
+    xs = rng.integers(0, 112, n)
+    d = numpy.stack([fr["data"][y:y + 16, x:x + 16] for y, x in zip(ys, xs)])
+    g = numpy.stack([fr["gamma"][y:y + 16, x:x + 16] for y, x in zip(ys, xs)])
+    return d.astype(numpy.float64), g.astype(numpy.float64)
+
+
+def _check(ctx, d, g, params, label):
+    mf = masks.to_fft_order(masks.otf_mask()).astype(numpy.float32)
+    ctx.set_otf_mask(mf)
+    d32, g32 = d.astype(numpy.float32), g.astype(numpy.float32)
+    u, it, st, cost = ctx.solve_tiles(d32, g32, 0.1, params)
+    ref = oracle.solve_tiles(d32, g32, mf, 0.1, f32=True)
+    f64, gn, un = oracle.eval_cost_f32(u, d32, g32, mf, 0.1)
+    rel = (f64 - ref["cost"]) / numpy.abs(ref["cost"])
+    _say(label, "max rel cost", float(rel.max()), "iters", float(it.mean()),
+         "status", dict(zip(*[x.tolist() for x in numpy.unique(st, return_counts=True)])),
+         "max |g|/max(1,|u|)", float(numpy.max(gn / numpy.maximum(1.0, un))))
+    assert numpy.all(st == 0), label
+    assert rel.max() <= 1e-5, label
+    assert numpy.all(gn <= 2e-5 * numpy.maximum(1.0, un)), label
+
+
+@pytest.mark.parametrize("scale", [1e-3, 1.0, 1e4, 1e6])
+def test_scaled_data_reach_the_oracle_minimiser(gpu_ctx, scale):
+    """Photon data times 1e-3 .. 1e6: the per-tile scale keeps the FP16
+    operands in range and precise at every magnitude."""
+    d, g = _tiles(11, 64)
+    _check(gpu_ctx, d * scale, g * scale, _lib.default_params(), "scale %g" % scale)
+
+
+def test_overflowing_gamma_transform_falls_back_to_fp32(gpu_ctx):
+    """gamma 1e4 times the data puts the learned start's gamma transform far
+    past FP16's range under the tile's data scale: the FP32 fallback must
+    produce the start the FP64 predictor gives."""
+    from ncs_amd import linstart
+    d, g = _tiles(14, 32)
+    d32 = d.astype(numpy.float32)
+    g32 = (1e4 * d.mean() * (g / g.mean())).astype(numpy.float32)
+    gpu_ctx.set_otf_mask(masks.to_fft_order(masks.otf_mask()).astype(numpy.float32))
+    coef = numpy.random.default_rng(3).uniform(-0.3, 0.3, linstart.N_COEF).astype(numpy.float32)
+    gpu_ctx.set_linear_start(coef)
+    p = _lib.default_params(start=_lib.START_LINEAR, max_iterations=0, fallback=0)
+    u, it, st, cost = gpu_ctx.solve_tiles(d32, g32, 0.1, p)
+    ref = linstart.predict(d32, g32, coef)
+    err = numpy.abs(u - ref).max(axis=(1, 2)) / numpy.abs(ref).max(axis=(1, 2))
+    _say("gamma 1e4 x data: start vs predictor max rel", float(err.max()))
+    assert numpy.all(numpy.isfinite(u)) and err.max() <= 2e-5
+
+
+def test_overflowing_gradient_falls_back_to_fp32(gpu_ctx):
+    """A start 1e5 below the data with gamma near 0 makes the likelihood
+    gradient about -1e5 per pixel, past FP16's range in the (unscaled)
+    gradient transform: the FP32 fallback must give a finite first step
+    that lowers the cost of every tile."""
+    d, g = _tiles(12, 32)
+    d32 = d.astype(numpy.float32)
+    g32 = numpy.full_like(d32, 1e-6)
+    u0 = (1e-5 * d).astype(numpy.float32)
+    mf = masks.to_fft_order(masks.otf_mask()).astype(numpy.float32)
+    gpu_ctx.set_otf_mask(mf)
+    import torch
+    dev = torch.device("cuda:0")
+    td, tg, tu0 = (torch.from_numpy(x).to(dev) for x in (d32, g32, u0))
+    tu = torch.empty_like(td)
+    ti = torch.empty(len(d), dtype=torch.int32, device=dev)
+    ts = torch.empty_like(ti)
+    p = _lib.default_params(fallback=0, max_iterations=1)
+    gpu_ctx.solve_tiles_device(td.data_ptr(), tg.data_ptr(), tu0.data_ptr(), tu.data_ptr(),
+                               ti.data_ptr(), ts.data_ptr(), None, len(d), 0.1, p,
+                               torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    u1 = tu.cpu().numpy()
+    f0, _, _ = oracle.eval_cost_f32(u0, d32, g32, mf, 0.1)
+    f1, _, _ = oracle.eval_cost_f32(u1, d32, g32, mf, 0.1)
+    _say("overflowing start: status", numpy.unique(ts.cpu().numpy()),
+         "cost ratio max", float(numpy.max(f1 / f0)))
+    assert numpy.all(numpy.isfinite(u1))
+    assert numpy.all(f1 < f0)
+
+
+def test_tight_tolerance_finishes_in_fp32(gpu_ctx):
+    """eps = 1e-7 is past the FP16 split's accuracy: the solver must switch
+    to FP32 and meet the tighter stop rule."""
+    d, g = _tiles(13, 32)
+    mf = masks.to_fft_order(masks.otf_mask()).astype(numpy.float32)
+    gpu_ctx.set_otf_mask(mf)
+    d32, g32 = d.astype(numpy.float32), g.astype(numpy.float32)
+    u, it, st, cost = gpu_ctx.solve_tiles(d32, g32, 0.1, _lib.default_params(epsilon=1e-7))
+    f64, gn, un = oracle.eval_cost_f32(u, d32, g32, mf, 0.1)
+    _say("eps 1e-7: status", numpy.unique(st), "max |g|/max(1,|u|)",
+         float(numpy.max(gn / numpy.maximum(1.0, un))))
+    assert numpy.all(st == 0)
+    assert numpy.all(gn <= 5e-7 * numpy.maximum(1.0, un))
