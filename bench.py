@@ -101,9 +101,9 @@ def flops_per_tile(iters, trials, m, warm=False):
         F_tile = F_eval (1 + T) + 512 T + (13 + 8m) 256 I
 
     with I iterations and T trials.  The solver executes less than this (the
-    NC Hessian is diagonal in its real-DFT basis: 8 MFMA per basis change,
-    two changes per iteration); the executed MFMA count is reported
-    separately from the profiler."""
+    NC Hessian is diagonal in its real-DFT basis: one basis change of 4 MFMA
+    16x16x32 f16 per evaluation, two per iteration); the executed MFMA
+    count is reported separately from the profiler."""
     f_eval = 102656.0
     # a warm start scores both starts: one more evaluation
     return (f_eval * (1.0 + trials + (1.0 if warm else 0.0)) + 512.0 * trials
@@ -780,8 +780,9 @@ def main():
                 "flop_model": "algorithmic: F_eval (1 + T) + 512 T + (13 + 8m) 256 I per tile, "
                               "F_eval = 102,656 (DFT-matrix evaluation), T = I",
                 "kernel": kernel, "kernel_ms": round(kms, 3),
-                "executed_mfma_tflops": (
-                    round(profiled("mfma_instructions_per_launch", kernel, path) * 2048.0
+                # the basis changes run MFMA 16x16x32 f16 (16,384 flops each)
+                "executed_mfma_tflops_f16": (
+                    round(profiled("mfma_instructions_per_launch", kernel, path) * 16384.0
                           / (kms * 1e-3) / 1e12, 2)
                     if profiled("mfma_instructions_per_launch", kernel, path) else None),
                 "flop_per_launch": flops,
