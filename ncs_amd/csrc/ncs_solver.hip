@@ -283,6 +283,7 @@ __device__ __forceinline__ bool all_finite(const float (&z)[4]) {
 
 // to_spec on the FP16 matrix cores: z = to_spec(x * s) / s (s a power of
 // two).  4 MFMA 16x16x32 f16; in FP32 when `exact` (uniform) is set.
+template <bool CHECK = true>
 __device__ __forceinline__ void to_spec16(const float (&x)[4], const LdsTables &T,
                                           const GlobalBasis &G, float s, float (&z)[4],
                                           bool exact) {
@@ -304,11 +305,12 @@ __device__ __forceinline__ void to_spec16(const float (&x)[4], const LdsTables &
   const float is = 1.0f / s;
 #pragma unroll
   for (int r = 0; r < 4; r++) z[r] = zz[r] * is;
-  if (!all_finite(z)) to_spec(x, G, z);
+  if (CHECK && !all_finite(z)) to_spec(x, G, z);
 }
 
 // to_pix on the FP16 matrix cores: x = to_pix(w * s) / s.  4 MFMA; in
 // FP32 when `exact` is set.
+template <bool CHECK = true>
 __device__ __forceinline__ void to_pix16(const float (&w)[4], const LdsTables &T,
                                          const GlobalBasis &G, float s, float (&x)[4],
                                          bool exact) {
@@ -330,7 +332,7 @@ __device__ __forceinline__ void to_pix16(const float (&w)[4], const LdsTables &T
   const float is = 1.0f / s;
 #pragma unroll
   for (int r = 0; r < 4; r++) x[r] = g[r] * is;
-  if (!all_finite(x)) to_pix(w, G, x);
+  if (CHECK && !all_finite(x)) to_pix(w, G, x);
 }
 
 // The NC Hessian K in the spectral basis: the diagonal 2 q_ee plus, for a
@@ -824,7 +826,7 @@ void solve_kernel(SolveArgs A) {
     bool exact = false;
     {
       float gll[4];
-      to_spec16(u, T, GB, st, zs, exact);
+      to_spec16<false>(u, T, GB, st, zs, exact);
       nc_spec<GEN>(zs, T, gncs);
       const bool warm = A.u0 != nullptr || A.lin_start;
       if (!warm) {
@@ -857,7 +859,7 @@ void solve_kernel(SolveArgs A) {
             pz[r] = sig * fmaf(la[r], zs[r] * isig, lb[r] * zg[r] * (isig * isig));
           if (lane == 0)  // DC: x0 has none; the bias term lives here
             pz[0] = sig * fmaf(lb[0], zg[0] * (isig * isig), 16.0f * A.lin_c);
-          to_pix16(pz, T, GB, st, wp, exact);
+          to_pix16<false>(pz, T, GB, st, wp, exact);
 #pragma unroll
           for (int r = 0; r < 4; r++) {
             w[r] = u[r] + wp[r];
@@ -1026,7 +1028,7 @@ void solve_kernel(SolveArgs A) {
                                (ut[3] != u[3]));
       gnc_exact = false;
       if ((k & 15) == 0) {  // periodic exact refresh of the NC gradient
-        to_spec16(ut, T, GB, st, zs, exact);
+        to_spec16<false>(ut, T, GB, st, zs, exact);
         nc_spec<GEN>(zs, T, gncn);
         gnc_exact = true;
       }
@@ -1057,7 +1059,7 @@ void solve_kernel(SolveArgs A) {
         if (!done && gg <= kF16Floor * kF16Floor * fmaxf(1.0f, uu)) exact = true;
         if (!done || gnc_exact) break;
         // confirm convergence with an exact NC gradient
-        to_spec16(ut, T, GB, st, zs, exact);
+        to_spec16<false>(ut, T, GB, st, zs, exact);
         nc_spec<GEN>(zs, T, gncn);
         gnc_exact = true;
       }
@@ -1148,7 +1150,7 @@ void solve_kernel(SolveArgs A) {
     // final cost at the returned point (when asked for), NC from an exact
     // transform of u
     if (A.cost != nullptr) {
-      to_spec16(u, T, GB, st, zs, exact);
+      to_spec16<false>(u, T, GB, st, zs, exact);
       nc_spec<GEN>(zs, T, gncs);
       float red[2];
       red[0] = ll_value_lane(u, gm, c, lin);
