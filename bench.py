@@ -383,11 +383,15 @@ def run_uneven(args):
         return (wall, kms, iters.cpu().numpy(), status.cpu().numpy(),
                 cost.cpu().numpy().astype(numpy.float64), [o.cpu().numpy() for o in out])
 
-    p = _lib.default_params(max_iterations=MAX_IT, history=0)  # the tuned depth
+    # the tuned schedule, explicitly: later autotuning replaces the
+    # context's cached one
+    p = _lib.default_params(max_iterations=MAX_IT, history=tune["history"],
+                            waves_per_cu=tune["waves_per_cu"],
+                            waves_per_group=tune["waves_per_group"])
     wall, kms, it, st, gc, outs = timed(p)
     # the learned linear start (fitted at alpha 0.1; the per-tile fallback
     # keeps the plain start wherever it scores lower), own schedule
-    lin = None
+    lin, lwall = None, None
     if not args.no_linear:
         from ncs_amd import linstart
         if os.path.exists(linstart.DEFAULT_COEF):
@@ -450,8 +454,8 @@ def run_uneven(args):
           % (rel.size, time.perf_counter() - t0, rel.max()), file=sys.stderr, flush=True)
     # headline: the faster path whose per-tile costs match the plain run's
     best_wall, best_path = wall, "plain"
-    if lin is not None and lin["matched_1e5"] and lin["ms_per_step"] < wall * 1e3:
-        best_wall, best_path = lin["ms_per_step"] * 1e-3, "linear"
+    if lin is not None and lin["matched_1e5"] and lwall < wall:
+        best_wall, best_path = lwall, "linear"
     line = {
         "metric": "corrected Mpix/s, uneven batch in one launch, 1 GPU",
         "value": round(pix / best_wall / 1e6, 2), "unit": "Mpix/s", "n_gpus": 1,
