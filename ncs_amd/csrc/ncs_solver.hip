@@ -910,7 +910,13 @@ void solve_kernel(SolveArgs A) {
 #pragma unroll
     for (int r = 0; r < 4; r++) ds[r] = -gs[r];
     float step = rsqrtf(gg);
-    if (A.precond && !done) {
+    // The spectral H0 (precond) models the likelihood curvature by its tile
+    // mean.  From a start far from the solution that mean can change by
+    // orders of magnitude along the way and the line search then fail; a
+    // tile whose search fails under it drops its history and continues with
+    // liblbfgs's scalar H0 from the same point.
+    bool spec_h0 = A.precond != 0;
+    if (spec_h0 && !done) {
       h0_spec(ds, T, kappa, alpha);
       step = 1.0f;
     }
@@ -937,7 +943,7 @@ void solve_kernel(SolveArgs A) {
         nh = 0;
 #pragma unroll
         for (int r = 0; r < 4; r++) ds[r] = -gs[r];
-        if (A.precond) {
+        if (spec_h0) {
           h0_spec(ds, T, kappa, alpha);
         } else {
           const float sc = rsqrtf(gg);
@@ -1009,6 +1015,14 @@ void solve_kernel(SolveArgs A) {
           }
         }
       }
+      if (ls != LS_ACCEPT && spec_h0) {  // retry with the scalar H0
+        spec_h0 = false;
+        nh = 0;
+#pragma unroll
+        for (int r = 0; r < 4; r++) ds[r] = -gs[r];
+        step = rsqrtf(gg);
+        continue;
+      }
       if (ls != LS_ACCEPT) {
         status = (ls == LS_FAIL_MINSTEP) ? NCS_MINIMUM_STEP
                  : (ls == LS_FAIL_INCREASE) ? NCS_INCREASING_GRADIENT
@@ -1045,7 +1059,7 @@ void solve_kernel(SolveArgs A) {
         float red[3] = {
             gnews[0] * gnews[0] + gnews[1] * gnews[1] + (gnews[2] * gnews[2] + gnews[3] * gnews[3]),
             ut[0] * ut[0] + ut[1] * ut[1] + (ut[2] * ut[2] + ut[3] * ut[3]),
-            A.precond ? curv_lane(c, ivt)
+            spec_h0 ? curv_lane(c, ivt)
                       : ((gnews[0] - gs[0]) * (gnews[0] - gs[0]) +
                          (gnews[1] - gs[1]) * (gnews[1] - gs[1]) +
                          ((gnews[2] - gs[2]) * (gnews[2] - gs[2]) +
@@ -1110,7 +1124,7 @@ void solve_kernel(SolveArgs A) {
           ds[3] = fmaf(-al[j], yv.w, ds[3]);
         }
       }
-      if (A.precond) {
+      if (spec_h0) {
         h0_spec(ds, T, kappa, alpha);
       } else {  // liblbfgs: H0 = (s.y / y.y) I
         const float hscale = (nh > 0) ? ys_last * rcpf(yy_last) : rsqrtf(gg);

@@ -177,3 +177,37 @@ def test_warm_start_from_the_solution_converges_immediately(gpu_ctx):
             assert abs(it2.mean() - it.mean()) <= 1.0  # fell back to the plain start
         rel = numpy.abs(c2.cpu().numpy() - cost) / numpy.abs(cost)
         assert rel.max() <= 1e-5
+
+
+@pytest.mark.parametrize("factor", [1e-5, 1e-2])
+def test_far_warm_start_reaches_the_minimiser(gpu_ctx, factor):
+    """A start far below the data with the fallback off: the spectral H0
+    models the curvature by its tile mean, which changes by orders of
+    magnitude on the way, so a failed line search hands the tile over to
+    liblbfgs's scalar H0.  Every tile must converge, to the oracle's cost."""
+    import torch
+    fr = sim.make_frame(12, (128, 128))
+    rng = numpy.random.default_rng(12)
+    d, g = _tiles_from_frame(fr, 32, rng)
+    d, g = d.astype(numpy.float32), g.astype(numpy.float32)
+    mf = masks.to_fft_order(masks.otf_mask()).astype(numpy.float32)
+    gpu_ctx.set_otf_mask(mf)
+    dev = torch.device("cuda:0")
+    td, tg = torch.from_numpy(d).to(dev), torch.from_numpy(g).to(dev)
+    tu0 = torch.from_numpy(factor * d).to(dev)
+    tu = torch.empty_like(td)
+    ti = torch.empty(len(d), dtype=torch.int32, device=dev)
+    ts = torch.empty_like(ti)
+    p = _lib.default_params(fallback=0, max_iterations=500)
+    gpu_ctx.solve_tiles_device(td.data_ptr(), tg.data_ptr(), tu0.data_ptr(), tu.data_ptr(),
+                               ti.data_ptr(), ts.data_ptr(), None, len(d), 0.1, p,
+                               torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    u, st = tu.cpu().numpy(), ts.cpu().numpy()
+    ref = oracle.solve_tiles(d, g, mf, 0.1, f32=True)
+    f64, gn, un = oracle.eval_cost_f32(u, d, g, mf, 0.1)
+    rel = (f64 - ref["cost"]) / numpy.abs(ref["cost"])
+    _say("far start", factor, "status", numpy.unique(st), "iters", ti.float().mean().item(),
+         "max rel cost", rel.max())
+    assert numpy.all(st == 0)
+    assert rel.max() <= 1e-5
