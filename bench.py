@@ -24,8 +24,9 @@ Two more workloads print their own line when asked for:
   --workload stream  200 frames of 2304^2 raw uint16 ADU (blinking filaments,
                      background drifting 10 -> 40 photons/pixel, fixed camera
                      maps) streamed frame by frame through two captured
-                     pipelines (ADU conversion, learned start, solve, stitch);
-                     sustained Mpix/s and per-frame latency p50 / p99.
+                     pipelines (ADU conversion, learned start, solve, stitch)
+                     with the host one frame ahead; sustained Mpix/s and
+                     per-frame latency (launch to consumption) p50 / p99.
 """
 import argparse
 import json
@@ -570,38 +571,45 @@ def run_stream(args, weights, lin_coef=None):
         warmstart.set_context_weights(weights)
         modes.append(("learned", _lib.START_LEARNED))
     check_frames = sorted({0, n // 2, n - 1})
+    depth, lag = args.stream_depth, args.stream_lag
+    if depth < 1 or lag < 0 or lag >= depth * _lib.PIPELINE_REPLAYS:
+        raise SystemExit("--stream-depth >= 1 and 0 <= --stream-lag < depth * %d"
+                         % _lib.PIPELINE_REPLAYS)
+    n_out = lag + 2  # an output buffer is reused only after its frame was consumed
     for label, start in modes:
         p = _lib.default_params(max_iterations=MAX_IT, start=start)
         pls = [_lib.Pipeline(ctx, [shape], [ALPHA], params=p, input_format=_lib.INPUT_ADU_U16,
-                             keep_tile_results=True) for _ in range(2)]
+                             keep_tile_results=True) for _ in range(depth)]
         for pl in pls:
             pl.set_camera(0, off32, gain32, var32)
-        outs = [torch.empty(shape, dtype=torch.float32, pin_memory=True) for _ in range(4)]
+        outs = [torch.empty(shape, dtype=torch.float32, pin_memory=True) for _ in range(n_out)]
         outs_np = [o.numpy() for o in outs]
         kept, costs = {}, {}
-        # warm-up replay on each pipeline (graph instantiation)
+        # warm-up replay on each pipeline (graph capture and instantiation)
         for pl in pls:
             pl.launch([adu[0]], [outs_np[0]])
             pl.wait()
         lat = numpy.zeros(n)
         t_launch = numpy.zeros(n)
+        seq = numpy.zeros(n, dtype=numpy.int64)
 
         def finish(k):
-            pl = pls[k % 2]
-            pl.wait()
+            # frame k is consumed `lag` launches after its own
+            pl = pls[k % depth]
+            pl.wait(seq[k])
             lat[k] = time.perf_counter() - t_launch[k]
             if k in check_frames:
-                kept[k] = outs_np[k % 4].copy()
-                costs[k] = pl.tile_results()[2]
+                kept[k] = outs_np[k % n_out].copy()
+                costs[k] = pl.tile_results(seq[k])[2]
 
         t0 = time.perf_counter()
         for k in range(n):
-            pl = pls[k % 2]
             t_launch[k] = time.perf_counter()
-            pl.launch([adu[k]], [outs_np[k % 4]])
-            if k >= 1:
-                finish(k - 1)
-        finish(n - 1)
+            seq[k] = pls[k % depth].launch([adu[k]], [outs_np[k % n_out]])
+            if k >= lag:
+                finish(k - lag)
+        for k in range(max(0, n - lag), n):
+            finish(k)
         wall = time.perf_counter() - t0
         for pl in pls:
             pl.close()
@@ -650,10 +658,11 @@ def run_stream(args, weights, lin_coef=None):
         "data": "synthetic time series (seed 5000): blinking WLC filaments p=0.3 per 1 um "
                 "segment, background 10->40 photons/px, fixed sCMOS maps, uint16 ADU",
         "config": {"workload": "%d frames of 2304x2304 uint16 ADU streamed frame by frame "
-                               "through two captured pipelines (H2D, ADU->photons, %s start, "
-                               "solve+stitch, D2H), alpha %.1f, m=5, eps 1e-5"
-                               % (n, best, ALPHA),
-                   "frames": n, "height": shape[0], "width": shape[1], "start": best},
+                               "through %d captured pipelines (H2D, ADU->photons, %s start, "
+                               "solve+stitch, D2H), host %d frames ahead, alpha %.1f, m=5, "
+                               "eps 1e-5" % (n, depth, best, lag, ALPHA),
+                   "frames": n, "height": shape[0], "width": shape[1], "start": best,
+                   "pipelines": depth, "frames_ahead": lag},
         "latency_ms_p50": summary(best)["latency_ms_p50"],
         "latency_ms_p99": summary(best)["latency_ms_p99"],
         "modes": {label: summary(label) for label in results},
@@ -690,6 +699,10 @@ def main():
                     help="run a 1 GiB device copy first (PMC byte-counter calibration)")
     ap.add_argument("--workload", choices=("batch", "uneven", "stream"), default="batch")
     ap.add_argument("--stream-frames", type=int, default=200)
+    ap.add_argument("--stream-depth", type=int, default=2,
+                    help="captured pipelines the stream workload alternates between")
+    ap.add_argument("--stream-lag", type=int, default=1,
+                    help="frames the host enqueues ahead of the one it consumes")
     ap.add_argument("--frames-cache", default=None,
                     help="read the batch from this .npz if present, else write it there")
     ap.add_argument("--schedule-cache", default=None,

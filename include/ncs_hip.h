@@ -248,15 +248,22 @@ int ncs_hip_autotune(ncs_ctx *ctx, const ncs_frame *frames, int n_frames,
 
 /* ---- captured streaming pipeline (no reference counterpart) ----
  *
- * A fixed batch shape is captured once as a hipGraph: upload of every frame
- * from host memory -> camera conversion (raw input) -> learned start (if
- * params->start is NCS_START_LEARNED) -> work-queue reset -> persistent solve
- * with the stitch fused -> download of every corrected frame (and, if asked,
- * of the per-tile results).  Each replay may point the copies at other host
- * buffers; pinned buffers (hipHostMalloc) make the copies asynchronous.  A
- * pipeline runs one replay at a time (ncs_hip_pipeline_launch waits for the
- * previous one); two pipelines overlap one frame's copies with another's
- * solve.  Gamma (or the camera maps) are uploaded once per pipeline. */
+ * The device work of a fixed batch shape is captured once as a hipGraph:
+ * camera conversion (raw input) -> learned start (if params->start is
+ * NCS_START_LEARNED) -> work-queue reset -> persistent solve with the stitch
+ * fused.  A replay is that graph between stream copies on the DMA engines:
+ * upload of every frame from host memory before it, download of every
+ * corrected frame (and, if asked, of the per-tile results) after it.  Each
+ * replay may name other host buffers at no cost; pinned buffers
+ * (hipHostMalloc) make the copies asynchronous.  Replays of one pipeline run
+ * one after the other in launch order (they share its device buffers);
+ * launching does not wait on the host for the previous replay, so the host
+ * can enqueue ahead of the device, up to NCS_PIPELINE_REPLAYS replays per
+ * pipeline.  Two pipelines overlap one frame's upload and solve with the
+ * other's download.  Gamma (or the camera maps) are uploaded once per
+ * pipeline. */
+
+#define NCS_PIPELINE_REPLAYS 4 /* replays of one pipeline in flight at most */
 
 #define NCS_INPUT_PHOTONS_F32 0 /* data in photo-electrons, float32 [H][W] */
 #define NCS_INPUT_ADU_U16 1     /* raw camera ADU, uint16 [H][W]; photons =
@@ -289,13 +296,25 @@ int ncs_hip_pipeline_set_camera(ncs_pipeline *pl, int frame, const float *offset
 /* Enqueue one replay on `stream` (0: the pipeline's own stream): inputs[i]
  * is frame i's host input, outputs[i] its host float32 output [H][W].  The
  * first launch needs both arrays; later ones may pass NULL to keep the
- * previous pointers. */
+ * previous pointers.  The buffers must stay valid until the replay is
+ * complete.  Replays are numbered 0, 1, ... in launch order.  A replay on
+ * another stream than the previous one waits for it on the device.  The
+ * host waits only when NCS_PIPELINE_REPLAYS replays are in flight (for the
+ * oldest). */
 int ncs_hip_pipeline_launch(ncs_pipeline *pl, const void *const *inputs,
                             float *const *outputs, void *stream);
-/* Wait for the last replay (its outputs are then in host memory). */
+/* Replays launched so far (the next replay's number); -1 for NULL. */
+long long ncs_hip_pipeline_launched(const ncs_pipeline *pl);
+/* Wait for every launched replay (outputs in host memory). */
 int ncs_hip_pipeline_wait(ncs_pipeline *pl);
-/* Per-tile results of the last replay (after wait), as for
- * ncs_hip_reduce_frames; needs keep_tile_results.  Any may be NULL. */
+/* Wait for replay `replay` and every earlier one. */
+int ncs_hip_pipeline_wait_replay(ncs_pipeline *pl, long long replay);
+/* Per-tile results of replay `replay`, one of the last NCS_PIPELINE_REPLAYS
+ * launched (waits for it), as for ncs_hip_reduce_frames; needs
+ * keep_tile_results.  Any output may be NULL. */
+int ncs_hip_pipeline_replay_results(ncs_pipeline *pl, long long replay, int *iterations,
+                                    int *status, float *cost);
+/* ncs_hip_pipeline_replay_results of the last replay launched. */
 int ncs_hip_pipeline_tile_results(ncs_pipeline *pl, int *iterations, int *status,
                                   float *cost);
 /* Tiles of one replay. */

@@ -33,6 +33,7 @@ NET_PARAMS = 19393
 LINEAR_START_PARAMS = 513
 INPUT_PHOTONS_F32 = 0
 INPUT_ADU_U16 = 1
+PIPELINE_REPLAYS = 4  # NCS_PIPELINE_REPLAYS
 
 
 class NCSHipError(RuntimeError):
@@ -130,7 +131,10 @@ SIGNATURES = {
     "ncs_hip_pipeline_set_gamma": (_i, [_vp, _i, _vp]),
     "ncs_hip_pipeline_set_camera": (_i, [_vp, _i, _vp, _vp, _vp]),
     "ncs_hip_pipeline_launch": (_i, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp]),
+    "ncs_hip_pipeline_launched": (_ll, [_vp]),
     "ncs_hip_pipeline_wait": (_i, [_vp]),
+    "ncs_hip_pipeline_wait_replay": (_i, [_vp, _ll]),
+    "ncs_hip_pipeline_replay_results": (_i, [_vp, _ll, _vp, _vp, _vp]),
     "ncs_hip_pipeline_tile_results": (_i, [_vp, _vp, _vp, _vp]),
     "ncs_hip_pipeline_tiles": (_ll, [_vp]),
 }
@@ -388,10 +392,12 @@ def context(device=0):
 
 
 class Pipeline(object):
-    """A captured upload -> solve -> download graph for a fixed batch shape
+    """Upload -> captured solve graph -> download for a fixed batch shape
     (ncs_hip_pipeline_*).  shapes: list of (height, width); alphas: one per
     frame; inputs and outputs are host numpy arrays (pinned memory makes the
-    copies asynchronous)."""
+    copies asynchronous).  Replays run in launch order, numbered from 0;
+    up to PIPELINE_REPLAYS may be in flight, and the arrays of each are kept
+    alive here until it is complete."""
 
     def __init__(self, ctx, shapes, alphas, params=None, geometry=GEOM_REFERENCE,
                  input_format=INPUT_PHOTONS_F32, keep_tile_results=False):
@@ -413,7 +419,8 @@ class Pipeline(object):
         self.shapes = [tuple(int(v) for v in s) for s in shapes]
         self.input_dtype = numpy.uint16 if input_format == INPUT_ADU_U16 else numpy.float32
         self.n_tiles = int(self.lib.ncs_hip_pipeline_tiles(h))
-        self._keep = []
+        self._keep = [None] * PIPELINE_REPLAYS
+        self._cur = [None, None]
 
     def close(self):
         if self.handle is not None:
@@ -445,7 +452,9 @@ class Pipeline(object):
               "ncs_hip_pipeline_set_camera")
 
     def launch(self, inputs=None, outputs=None, stream_ptr=None):
-        """Enqueue one replay.  The arrays must stay alive until wait()."""
+        """Enqueue one replay and return its number.  None keeps the previous
+        replay's arrays.  The host waits only for the oldest replay when
+        PIPELINE_REPLAYS are in flight."""
         n = len(self.shapes)
         ins = outs = None
         if inputs is not None:
@@ -456,17 +465,42 @@ class Pipeline(object):
             for i, a in enumerate(outputs):
                 self._check_frame(i, a, numpy.float32)
             outs = (ctypes.c_void_p * n)(*[a.ctypes.data for a in outputs])
-        check(self.lib.ncs_hip_pipeline_launch(self.handle, ins, outs, stream_ptr),
-              "ncs_hip_pipeline_launch")
-        self._keep = [inputs, outputs]
+        seq = int(self.lib.ncs_hip_pipeline_launched(self.handle))
+        rc = self.lib.ncs_hip_pipeline_launch(self.handle, ins, outs, stream_ptr)
+        if int(self.lib.ncs_hip_pipeline_launched(self.handle)) > seq:
+            # enqueued (even if part of it failed): its arrays stay alive
+            # until the launch that reuses its slot has waited for it
+            if inputs is not None:
+                self._cur[0] = list(inputs)
+            if outputs is not None:
+                self._cur[1] = list(outputs)
+            self._keep[seq % PIPELINE_REPLAYS] = tuple(self._cur)
+        check(rc, "ncs_hip_pipeline_launch")
+        return seq
 
-    def wait(self):
-        check(self.lib.ncs_hip_pipeline_wait(self.handle), "ncs_hip_pipeline_wait")
+    def launched(self):
+        return int(self.lib.ncs_hip_pipeline_launched(self.handle))
 
-    def tile_results(self):
+    def wait(self, replay=None):
+        """Wait for replay `replay` and every earlier one (None: all)."""
+        if replay is None:
+            check(self.lib.ncs_hip_pipeline_wait(self.handle), "ncs_hip_pipeline_wait")
+        else:
+            check(self.lib.ncs_hip_pipeline_wait_replay(self.handle, int(replay)),
+                  "ncs_hip_pipeline_wait_replay")
+
+    def tile_results(self, replay=None):
+        """(iterations, status, cost) per tile of replay `replay` (one of the
+        last PIPELINE_REPLAYS launched; None: the last); waits for it."""
         it = numpy.zeros(self.n_tiles, numpy.int32)
         st = numpy.zeros(self.n_tiles, numpy.int32)
         co = numpy.zeros(self.n_tiles, numpy.float32)
-        check(self.lib.ncs_hip_pipeline_tile_results(self.handle, _ptr(it), _ptr(st), _ptr(co)),
-              "ncs_hip_pipeline_tile_results")
+        if replay is None:
+            check(self.lib.ncs_hip_pipeline_tile_results(self.handle, _ptr(it), _ptr(st),
+                                                         _ptr(co)),
+                  "ncs_hip_pipeline_tile_results")
+        else:
+            check(self.lib.ncs_hip_pipeline_replay_results(self.handle, int(replay), _ptr(it),
+                                                           _ptr(st), _ptr(co)),
+                  "ncs_hip_pipeline_replay_results")
         return it, st, co

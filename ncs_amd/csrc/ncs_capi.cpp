@@ -913,6 +913,10 @@ extern "C" int ncs_hip_autotune(ncs_ctx *ctx, const ncs_frame *frames, int n_fra
 // captured streaming pipeline
 // ---------------------------------------------------------------------------
 
+// Replays of one pipeline in flight at once; each has its own completion
+// event and host copy of the per-tile results.
+constexpr int kReplaySlots = NCS_PIPELINE_REPLAYS;
+
 struct ncs_pipeline {
   ncs_ctx *ctx = nullptr;
   int n = 0;
@@ -925,13 +929,14 @@ struct ncs_pipeline {
   long long total = 0;
   bool pad = false, general = false, keep = false, learned = false;
   Schedule sched;
-  DevBuf raw, data, gamma, out, offset, gain, frames, queue, iters, status, cost, u0;
-  int *h_iters = nullptr, *h_status = nullptr;
-  float *h_cost = nullptr;
+  DevBuf raw, data, gamma, out, offset, gain, frames, queue, results, u0;
+  char *h_res[kReplaySlots] = {};  // per slot: iterations, status, cost
   std::vector<char> gamma_set;
   hipStream_t stream = nullptr;
-  hipEvent_t done = nullptr;
-  bool pending = false;
+  hipEvent_t done[kReplaySlots] = {};
+  long long launched = 0;  // replays enqueued
+  long long waited = 0;    // replays known to be complete
+  hipStream_t last_stream = nullptr;
   hipGraphExec_t exec = nullptr;
   std::vector<const void *> cur_in;
   std::vector<float *> cur_out;
@@ -940,15 +945,16 @@ struct ncs_pipeline {
 extern "C" void ncs_hip_pipeline_destroy(ncs_pipeline *pl) {
   if (!pl) return;
   if (pl->ctx) (void)hipSetDevice(pl->ctx->device);
-  if (pl->pending) (void)hipEventSynchronize(pl->done);
+  if (pl->launched > pl->waited)
+    (void)hipEventSynchronize(pl->done[(pl->launched - 1) % kReplaySlots]);
   if (pl->exec) (void)hipGraphExecDestroy(pl->exec);
   DevBuf *bufs[] = {&pl->raw, &pl->data, &pl->gamma, &pl->out, &pl->offset, &pl->gain,
-                    &pl->frames, &pl->queue, &pl->iters, &pl->status, &pl->cost, &pl->u0};
+                    &pl->frames, &pl->queue, &pl->results, &pl->u0};
   for (DevBuf *b : bufs) b->release();
-  if (pl->h_iters) (void)hipHostFree(pl->h_iters);
-  if (pl->h_status) (void)hipHostFree(pl->h_status);
-  if (pl->h_cost) (void)hipHostFree(pl->h_cost);
-  if (pl->done) (void)hipEventDestroy(pl->done);
+  for (int k = 0; k < kReplaySlots; k++) {
+    if (pl->h_res[k]) (void)hipHostFree(pl->h_res[k]);
+    if (pl->done[k]) (void)hipEventDestroy(pl->done[k]);
+  }
   if (pl->stream) (void)hipStreamDestroy(pl->stream);
   delete pl;
 }
@@ -1018,15 +1024,12 @@ extern "C" int ncs_hip_pipeline_create(ncs_ctx *ctx, const ncs_pipeline_frame *f
     HIPCHK(pl->gain.ensure(fb));
   }
   HIPCHK(pl->queue.ensure(kQueueBytes));
-  HIPCHK(pl->iters.ensure((size_t)pl->total * 4));
-  HIPCHK(pl->status.ensure((size_t)pl->total * 4));
-  HIPCHK(pl->cost.ensure((size_t)pl->total * 4));
+  // iterations, status and cost back to back: one download per replay
+  HIPCHK(pl->results.ensure((size_t)pl->total * 12));
   if (pl->learned) HIPCHK(pl->u0.ensure((size_t)pl->total * kTilePix * sizeof(float)));
-  if (pl->keep) {
-    HIPCHK(hipHostMalloc((void **)&pl->h_iters, (size_t)pl->total * 4, hipHostMallocDefault));
-    HIPCHK(hipHostMalloc((void **)&pl->h_status, (size_t)pl->total * 4, hipHostMallocDefault));
-    HIPCHK(hipHostMalloc((void **)&pl->h_cost, (size_t)pl->total * 4, hipHostMallocDefault));
-  }
+  if (pl->keep)
+    for (int k = 0; k < kReplaySlots; k++)
+      HIPCHK(hipHostMalloc((void **)&pl->h_res[k], (size_t)pl->total * 12, hipHostMallocDefault));
   for (int i = 0; i < n_frames; i++) {
     fd[i].data = pl->data.as<float>() + pl->off[i];
     fd[i].gamma = pl->gamma.as<float>() + pl->off[i];
@@ -1035,19 +1038,25 @@ extern "C" int ncs_hip_pipeline_create(ncs_ctx *ctx, const ncs_pipeline_frame *f
   HIPCHK(pl->frames.ensure(fd.size() * sizeof(FrameDev)));
   HIPCHK(hipMemcpy(pl->frames.p, fd.data(), fd.size() * sizeof(FrameDev), hipMemcpyHostToDevice));
   HIPCHK(hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking));
-  HIPCHK(hipEventCreateWithFlags(&pl->done, hipEventDisableTiming));
+  for (int k = 0; k < kReplaySlots; k++)
+    HIPCHK(hipEventCreateWithFlags(&pl->done[k], hipEventDisableTiming));
   pl->gamma_set.assign(n_frames, 0);
   pl->sched = resolve_schedule(ctx, params, pl->pad);
   *out = pl.release();
   return NCS_OK;
 }
 
-static int pipeline_idle(ncs_pipeline *pl) {
-  if (pl->pending) {
-    HIPCHK(hipEventSynchronize(pl->done));
-    pl->pending = false;
-  }
+// Waits on the host for replay `seq` (launch order).  Replays of one
+// pipeline run in launch order, so every earlier one is complete too.
+static int wait_replay(ncs_pipeline *pl, long long seq) {
+  if (seq < pl->waited) return NCS_OK;
+  HIPCHK(hipEventSynchronize(pl->done[seq % kReplaySlots]));
+  pl->waited = seq + 1;
   return NCS_OK;
+}
+
+static int pipeline_idle(ncs_pipeline *pl) {
+  return pl->launched ? wait_replay(pl, pl->launched - 1) : NCS_OK;
 }
 
 extern "C" int ncs_hip_pipeline_set_gamma(ncs_pipeline *pl, int frame, const float *gamma) {
@@ -1083,20 +1092,20 @@ extern "C" int ncs_hip_pipeline_set_camera(ncs_pipeline *pl, int frame, const fl
   return NCS_OK;
 }
 
-// One replay captured on the pipeline's stream as a single chain: upload of
-// every frame, camera conversion, learned start, queue reset and solve,
-// download of every frame and of the per-tile results.  Stream order becomes
-// the graph's edges.
-static int capture_replay(ncs_pipeline *pl, const void *const *inputs, float *const *outputs,
-                          hipGraph_t *out) {
+// The device part of a replay, captured once on the pipeline's stream as a
+// single chain: camera conversion, learned start, queue reset and solve.
+// The host transfers stay outside the graph: a copy node of a graph runs as
+// a blit kernel on the compute units, where it holds wave slots the
+// persistent solver's statically assigned waves need (a 2304x2304 solve
+// overlapping one took 5x its time alone) and moves data one PCIe direction
+// at a time, while a stream copy runs on the DMA engines, uploads and
+// downloads of different pipelines at once.  With the host pointers out of
+// the graph, new buffers need no capture or update.
+static int capture_replay(ncs_pipeline *pl, hipGraph_t *out) {
   ncs_ctx *ctx = pl->ctx;
   hipStream_t s = pl->stream;
   HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
   hipError_t e = hipSuccess;
-  char *dev_in = pl->input == NCS_INPUT_ADU_U16 ? pl->raw.as<char>() : pl->data.as<char>();
-  for (int i = 0; i < pl->n && e == hipSuccess; i++)
-    e = hipMemcpyAsync(dev_in + pl->off[i] * pl->elem, inputs[i],
-                       frame_pixels(pl->shape[i]) * pl->elem, hipMemcpyHostToDevice, s);
   if (pl->input == NCS_INPUT_ADU_U16) {
     for (int i = 0; i < pl->n && e == hipSuccess; i++) {
       const size_t o = pl->off[i];
@@ -1114,18 +1123,11 @@ static int capture_replay(ncs_pipeline *pl, const void *const *inputs, float *co
   }
   if (e == hipSuccess) {
     const SolveArgs a = make_args(ctx, pl->frames.as<FrameDev>(), pl->n, pl->total, &pl->params,
-                                  u0, nullptr, pl->iters.as<int>(), pl->status.as<int>(),
-                                  pl->cost.as<float>(), pl->queue.as<unsigned int>());
+                                  u0, nullptr, pl->results.as<int>(),
+                                  pl->results.as<int>() + pl->total,
+                                  pl->results.as<float>() + 2 * pl->total,
+                                  pl->queue.as<unsigned int>());
     e = launch_schedule(ctx, pl->sched, pl->pad, pl->total, s, a);
-  }
-  for (int i = 0; i < pl->n && e == hipSuccess; i++)
-    e = hipMemcpyAsync(outputs[i], pl->out.as<float>() + pl->off[i],
-                       frame_pixels(pl->shape[i]) * sizeof(float), hipMemcpyDeviceToHost, s);
-  if (e == hipSuccess && pl->keep) {
-    const size_t b = (size_t)pl->total * 4;
-    e = hipMemcpyAsync(pl->h_iters, pl->iters.p, b, hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(pl->h_status, pl->status.p, b, hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(pl->h_cost, pl->cost.p, b, hipMemcpyDeviceToHost, s);
   }
   hipGraph_t g = nullptr;
   hipError_t e2 = hipStreamEndCapture(s, &g);
@@ -1138,40 +1140,41 @@ static int capture_replay(ncs_pipeline *pl, const void *const *inputs, float *co
   return NCS_OK;
 }
 
-// Captures the replay for these host buffers and instantiates it, or updates
-// the existing executable graph in place (same topology, new pointers).
-static int build_pipeline_graph(ncs_pipeline *pl, const void *const *inputs,
-                                float *const *outputs) {
+static int build_pipeline_graph(ncs_pipeline *pl) {
   hipGraph_t graph = nullptr;
-  int rc = capture_replay(pl, inputs, outputs, &graph);
+  int rc = capture_replay(pl, &graph);
   if (rc) return rc;
-  hipError_t e = hipSuccess;
-  bool updated = false;
-  if (pl->exec) {
-    hipGraphNode_t err_node = nullptr;
-    hipGraphExecUpdateResult res;
-    updated = hipGraphExecUpdate(pl->exec, graph, &err_node, &res) == hipSuccess &&
-              res == hipGraphExecUpdateSuccess;
-    if (!updated) {
-      (void)hipGetLastError();
-      (void)hipGraphExecDestroy(pl->exec);
-      pl->exec = nullptr;
-    }
-  }
-  if (!updated) e = hipGraphInstantiate(&pl->exec, graph, nullptr, nullptr, 0);
+  hipError_t e = hipGraphInstantiate(&pl->exec, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
   if (e != hipSuccess) pl->exec = nullptr;
   HIPCHK(e);
   pl->general = pl->ctx->general;
-  pl->cur_in.assign(inputs, inputs + pl->n);
-  pl->cur_out.assign(outputs, outputs + pl->n);
   return NCS_OK;
+}
+
+// Enqueues one replay on s: upload of every frame, the graph, download of
+// every frame and of the per-tile results into the replay's slot.
+static hipError_t enqueue_replay(ncs_pipeline *pl, hipStream_t s, int slot) {
+  char *dev_in = pl->input == NCS_INPUT_ADU_U16 ? pl->raw.as<char>() : pl->data.as<char>();
+  hipError_t e = hipSuccess;
+  for (int i = 0; i < pl->n && e == hipSuccess; i++)
+    e = hipMemcpyAsync(dev_in + pl->off[i] * pl->elem, pl->cur_in[i],
+                       frame_pixels(pl->shape[i]) * pl->elem, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipGraphLaunch(pl->exec, s);
+  for (int i = 0; i < pl->n && e == hipSuccess; i++)
+    e = hipMemcpyAsync(pl->cur_out[i], pl->out.as<float>() + pl->off[i],
+                       frame_pixels(pl->shape[i]) * sizeof(float), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess && pl->keep) {
+    e = hipMemcpyAsync(pl->h_res[slot], pl->results.p, (size_t)pl->total * 12,
+                       hipMemcpyDeviceToHost, s);
+  }
+  return e;
 }
 
 extern "C" int ncs_hip_pipeline_launch(ncs_pipeline *pl, const void *const *inputs,
                                        float *const *outputs, void *stream) {
   if (!pl) return fail(NCS_E_ARG, "bad arguments");
-  if (!pl->exec && (!inputs || !outputs))
+  if (pl->cur_in.empty() && (!inputs || !outputs))
     return fail(NCS_E_ARG, "the first launch needs inputs and outputs");
   for (int i = 0; i < pl->n; i++) {
     if ((inputs && !inputs[i]) || (outputs && !outputs[i]))
@@ -1183,26 +1186,39 @@ extern "C" int ncs_hip_pipeline_launch(ncs_pipeline *pl, const void *const *inpu
   if (!ctx->has_mask) return fail(NCS_E_NOMASK, "no OTF mask set");
   if (pl->learned && !ctx->has_net) return fail(NCS_E_NOWEIGHTS, "no network weights set");
   HIPCHK(set_device(ctx));
-  int rc = pipeline_idle(pl);
-  if (rc) return rc;
   if (pl->exec && ctx->general != pl->general)
     return fail(NCS_E_MASK, "the kind of the OTF mask changed since the pipeline was "
                 "captured; create a new pipeline");
-  bool same = pl->exec != nullptr;
-  for (int i = 0; i < pl->n && same; i++)
-    same = (!inputs || inputs[i] == pl->cur_in[i]) && (!outputs || outputs[i] == pl->cur_out[i]);
-  if (!same) {
-    std::vector<const void *> in(pl->cur_in);
-    std::vector<float *> outv(pl->cur_out);
-    if (inputs) in.assign(inputs, inputs + pl->n);
-    if (outputs) outv.assign(outputs, outputs + pl->n);
-    rc = build_pipeline_graph(pl, in.data(), outv.data());
+  int rc;
+  if (!pl->exec) {
+    rc = build_pipeline_graph(pl);
     if (rc) return rc;
   }
+  const long long seq = pl->launched;
+  const int slot = (int)(seq % kReplaySlots);
+  // the slot's last replay must be complete before its event and results
+  // are reused
+  if (seq >= kReplaySlots) {
+    rc = wait_replay(pl, seq - kReplaySlots);
+    if (rc) return rc;
+  }
+  if (inputs) pl->cur_in.assign(inputs, inputs + pl->n);
+  if (outputs) pl->cur_out.assign(outputs, outputs + pl->n);
   hipStream_t s = stream ? (hipStream_t)stream : pl->stream;
-  HIPCHK(hipGraphLaunch(pl->exec, s));
-  HIPCHK(hipEventRecord(pl->done, s));
-  pl->pending = true;
+  hipError_t e = hipSuccess;
+  // the device buffers are shared by every replay: one enqueued on another
+  // stream waits on the device for the previous one
+  if (seq > pl->waited && s != pl->last_stream)
+    e = hipStreamWaitEvent(s, pl->done[(seq - 1) % kReplaySlots], 0);
+  if (e == hipSuccess) e = enqueue_replay(pl, s, slot);
+  // whatever was enqueued is covered by the replay's event
+  hipError_t e2 = hipEventRecord(pl->done[slot], s);
+  if (e2 == hipSuccess) {
+    pl->launched = seq + 1;
+    pl->last_stream = s;
+  }
+  HIPCHK(e);
+  HIPCHK(e2);
   return NCS_OK;
 }
 
@@ -1212,18 +1228,40 @@ extern "C" int ncs_hip_pipeline_wait(ncs_pipeline *pl) {
   return pipeline_idle(pl);
 }
 
+extern "C" int ncs_hip_pipeline_wait_replay(ncs_pipeline *pl, long long replay) {
+  if (!pl || replay < 0 || replay >= pl->launched)
+    return fail(NCS_E_ARG, "bad arguments or a replay not launched");
+  HIPCHK(set_device(pl->ctx));
+  return wait_replay(pl, replay);
+}
+
+extern "C" int ncs_hip_pipeline_replay_results(ncs_pipeline *pl, long long replay,
+                                               int *iterations, int *status, float *cost) {
+  if (!pl) return fail(NCS_E_ARG, "bad arguments");
+  if (!pl->keep) return fail(NCS_E_ARG, "the pipeline was created without tile results");
+  if (replay < 0 || replay >= pl->launched || replay < pl->launched - kReplaySlots)
+    return fail(NCS_E_ARG, "replay " + std::to_string(replay) + " is not among the last " +
+                               std::to_string(kReplaySlots) + " launched");
+  HIPCHK(set_device(pl->ctx));
+  int rc = wait_replay(pl, replay);
+  if (rc) return rc;
+  const size_t b = (size_t)pl->total * 4;
+  const char *h = pl->h_res[replay % kReplaySlots];
+  if (iterations) std::memcpy(iterations, h, b);
+  if (status) std::memcpy(status, h + b, b);
+  if (cost) std::memcpy(cost, h + 2 * b, b);
+  return NCS_OK;
+}
+
 extern "C" int ncs_hip_pipeline_tile_results(ncs_pipeline *pl, int *iterations, int *status,
                                              float *cost) {
   if (!pl) return fail(NCS_E_ARG, "bad arguments");
-  if (!pl->keep) return fail(NCS_E_ARG, "the pipeline was created without tile results");
-  HIPCHK(set_device(pl->ctx));
-  int rc = pipeline_idle(pl);
-  if (rc) return rc;
-  const size_t b = (size_t)pl->total * 4;
-  if (iterations) std::memcpy(iterations, pl->h_iters, b);
-  if (status) std::memcpy(status, pl->h_status, b);
-  if (cost) std::memcpy(cost, pl->h_cost, b);
-  return NCS_OK;
+  if (!pl->launched) return fail(NCS_E_ARG, "no replay launched");
+  return ncs_hip_pipeline_replay_results(pl, pl->launched - 1, iterations, status, cost);
+}
+
+extern "C" long long ncs_hip_pipeline_launched(const ncs_pipeline *pl) {
+  return pl ? pl->launched : -1;
 }
 
 extern "C" long long ncs_hip_pipeline_tiles(const ncs_pipeline *pl) {

@@ -20,7 +20,11 @@
 //   grad NC = R^T W R,  W = 2 qe * Z + qo * partner(Z),
 // four 16x16x16 FP32 GEMMs = 16 MFMA 16x16x4 per application, every operand
 // taken straight from the register it sits in (no LDS, no shuffles; the
-// general-mask partner is one lane permutation, lane ^ 40).
+// general-mask partner is one lane permutation, lane ^ 40).  The evaluation
+// kernel works this way.  The solver keeps its state in the R basis, where
+// K is diagonal, and pays only the basis changes: 4 MFMA 16x16x32 f16 each
+// on an FP16 hi/lo split of both operands, or 8 FP32 MFMA where FP16 is not
+// accurate enough (a non-finite result, or a gradient near the stop rule).
 //
 // NC is quadratic, so along a search direction d:
 //   NC(u + t d) = NC(u) + t (gnc . d) + t^2/2 (d . K d),  gnc(u + t d) = gnc + t K d.
@@ -816,7 +820,7 @@ void solve_kernel(SolveArgs A) {
     // lives in the orthonormal real-DFT basis, where K and the spectral H0
     // are diagonal: each iteration needs one transform to pixels (the
     // direction, for the likelihood trial) and one back (the likelihood
-    // gradient), 8 MFMA each.  Dot products are basis-independent.
+    // gradient).  Dot products are basis-independent.
     float gs[4], gncs[4], zs[4], iv[4];  // spectral g, K u; pixel 1/(u+gamma)
     float f = 0.0f;  // f(u) - f(u_start): only differences drive the search
     // Basis changes run on the FP16 matrix cores (22-bit operands) until the

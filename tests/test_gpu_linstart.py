@@ -1,6 +1,6 @@
 """GPU checks of the learned linear start fused into the solve kernel.
 
-The start the kernel computes (three 8-MFMA basis changes) must equal the
+The start the kernel computes (three basis changes on the matrix cores) must equal the
 FP64 numpy predictor of ncs_amd.linstart; solves from it must reach the
 plain start's per-tile costs within 1e-5; coefficients that make the start
 worse fall back to the plain start per tile."""

@@ -152,7 +152,7 @@ def test_pipeline_blocks_geometry_uneven_frames(otf):
 
 
 def test_pipeline_rejects_missing_gamma_and_bad_buffers(otf):
-    pl = _lib.Pipeline(otf, [(32, 32)], [0.1])
+    pl = _lib.Pipeline(otf, [(32, 32)], [0.1], keep_tile_results=True)
     a = numpy.zeros((32, 32), numpy.float32)
     with pytest.raises(_lib.NCSHipError):
         pl.launch([a], [a.copy()])  # gamma not set
@@ -161,6 +161,55 @@ def test_pipeline_rejects_missing_gamma_and_bad_buffers(otf):
         pl.launch([numpy.zeros((31, 32), numpy.float32)], [a])
     with pytest.raises(_lib.NCSHipError):
         pl.launch(None, None)  # the first launch needs buffers
+    with pytest.raises(_lib.NCSHipError):
+        pl.tile_results()  # nothing launched
+    with pytest.raises(_lib.NCSHipError):
+        pl.wait(0)
+    assert pl.launched() == 0
+    pl.close()
+
+
+def test_replays_in_flight_on_two_streams_match_eager(otf):
+    """Seven replays enqueued without host waits, alternating between the
+    pipeline's stream and a second one, each with its own input and output:
+    the device-side ordering must keep every replay's image and tile results
+    those of the eager batch on its input.  Seven is past the four replay
+    slots, so the launch that reuses a slot waits for its previous replay."""
+    import torch
+    ctx = otf
+    pin = _Pinned()
+    shape = (96, 130)
+    f = sim.make_frame(61, shape)
+    p = _lib.default_params()
+    n = 7
+    rng = numpy.random.default_rng(8)
+    ins = [pin(shape, numpy.float32) for _ in range(n)]
+    outs = [pin(shape, numpy.float32) for _ in range(n)]
+    refs = []
+    for k in range(n):
+        ins[k][...] = numpy.maximum(f["data"] * rng.uniform(0.5, 1.5, shape), 1e-6)
+        refs.append(_eager(ctx, [ins[k].copy()], [f["gamma"]], [0.1], p))
+    pl = _lib.Pipeline(ctx, [shape], [0.1], params=p, keep_tile_results=True)
+    pl.set_gamma(0, f["gamma"])
+    other = torch.cuda.Stream()
+    seqs = [pl.launch([ins[k]], [outs[k]], None if k % 2 == 0 else other.cuda_stream)
+            for k in range(n)]
+    assert seqs == list(range(n)) and pl.launched() == n
+    _say("launched", n, "replays")
+    for k in range(n - _lib.PIPELINE_REPLAYS, n):
+        it, st, co = pl.tile_results(k)
+        assert numpy.array_equal(co, refs[k][3]) and numpy.array_equal(it, refs[k][1]), k
+        assert numpy.array_equal(st, refs[k][2]), k
+    with pytest.raises(_lib.NCSHipError):
+        pl.tile_results(n - _lib.PIPELINE_REPLAYS - 1)  # its slot was reused
+    pl.wait()
+    for k in range(n):
+        assert numpy.array_equal(outs[k], refs[k][0][0]), k
+    # a replay with new outputs only keeps the last inputs
+    out2 = pin(shape, numpy.float32)
+    seq = pl.launch(None, [out2])
+    pl.wait(seq)
+    assert numpy.array_equal(out2, refs[n - 1][0][0])
     pl.close()
 
 
