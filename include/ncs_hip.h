@@ -264,6 +264,7 @@ int ncs_hip_autotune(ncs_ctx *ctx, const ncs_frame *frames, int n_frames,
  * pipeline. */
 
 #define NCS_PIPELINE_REPLAYS 4 /* replays of one pipeline in flight at most */
+#define NCS_MAX_GAIN_LEVELS 4  /* gain amplifiers of one camera at most */
 
 #define NCS_INPUT_PHOTONS_F32 0 /* data in photo-electrons, float32 [H][W] */
 #define NCS_INPUT_ADU_U16 1     /* raw camera ADU, uint16 [H][W]; photons =
@@ -293,6 +294,21 @@ int ncs_hip_pipeline_set_gamma(ncs_pipeline *pl, int frame, const float *gamma);
  * gain^2 (denoisetools.py:266).  Synchronous. */
 int ncs_hip_pipeline_set_camera(ncs_pipeline *pl, int frame, const float *offset,
                                 const float *gain, const float *variance);
+/* Camera maps of frame i for an sCMOS with several gain amplifiers
+ * (NCS_INPUT_ADU_U16; replaces matlab/genmaps.m:29-62 and its use in
+ * matlab/multimaps_example.m).  n_levels (1..NCS_MAX_GAIN_LEVELS) gain and
+ * read-variance maps, host float32 [n_levels][H][W], and n_levels - 1
+ * ascending ADU division points (float64; may be NULL for one level).  At
+ * every replay each pixel picks level L = the number of division points
+ * <= its ADU, then data = (ADU - offset)/gain_L (values <= 0 set to 1e-6)
+ * and gamma = variance_L/gain_L^2.  One level is ncs_hip_pipeline_set_camera.
+ * Synchronous; a change of levels or division points captures the graph
+ * again at the next launch.  Division points that descend are rejected
+ * (genmaps.m would keep the previous frame's maps for the pixels no level
+ * covers). */
+int ncs_hip_pipeline_set_camera_levels(ncs_pipeline *pl, int frame, int n_levels,
+                                       const double *points, const float *offset,
+                                       const float *gains, const float *variances);
 /* Enqueue one replay on `stream` (0: the pipeline's own stream): inputs[i]
  * is frame i's host input, outputs[i] its host float32 output [H][W].  The
  * first launch needs both arrays; later ones may pass NULL to keep the

@@ -34,6 +34,7 @@ LINEAR_START_PARAMS = 513
 INPUT_PHOTONS_F32 = 0
 INPUT_ADU_U16 = 1
 PIPELINE_REPLAYS = 4  # NCS_PIPELINE_REPLAYS
+MAX_GAIN_LEVELS = 4  # NCS_MAX_GAIN_LEVELS
 
 
 class NCSHipError(RuntimeError):
@@ -130,6 +131,7 @@ SIGNATURES = {
     "ncs_hip_pipeline_destroy": (None, [_vp]),
     "ncs_hip_pipeline_set_gamma": (_i, [_vp, _i, _vp]),
     "ncs_hip_pipeline_set_camera": (_i, [_vp, _i, _vp, _vp, _vp]),
+    "ncs_hip_pipeline_set_camera_levels": (_i, [_vp, _i, _i, _vp, _vp, _vp, _vp]),
     "ncs_hip_pipeline_launch": (_i, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp]),
     "ncs_hip_pipeline_launched": (_ll, [_vp]),
     "ncs_hip_pipeline_wait": (_i, [_vp]),
@@ -450,6 +452,24 @@ class Pipeline(object):
             self._check_frame(i, m, numpy.float32)
         check(self.lib.ncs_hip_pipeline_set_camera(self.handle, int(i), *[_ptr(m) for m in maps]),
               "ncs_hip_pipeline_set_camera")
+
+    def set_camera_levels(self, i, points, offset, gains, variances):
+        """Camera maps of frame i for several gain amplifiers: gains and
+        variances [levels][H][W], `points` the levels - 1 ascending ADU
+        division points (matlab/genmaps.m:29-62)."""
+        off = numpy.ascontiguousarray(offset, dtype=numpy.float32)
+        self._check_frame(i, off, numpy.float32)
+        g = numpy.ascontiguousarray(gains, dtype=numpy.float32)
+        v = numpy.ascontiguousarray(variances, dtype=numpy.float32)
+        levels = g.shape[0] if g.ndim == 3 else 0
+        if g.shape != (levels,) + self.shapes[i] or v.shape != g.shape:
+            raise ValueError("gains and variances must be [levels][%d][%d]" % self.shapes[i])
+        pts = numpy.ascontiguousarray(points, dtype=numpy.float64).reshape(-1)
+        if pts.size != levels - 1:
+            raise ValueError("%d levels take %d division points" % (levels, levels - 1))
+        check(self.lib.ncs_hip_pipeline_set_camera_levels(
+            self.handle, int(i), int(levels), pts.ctypes.data if pts.size else None, _ptr(off),
+            _ptr(g), _ptr(v)), "ncs_hip_pipeline_set_camera_levels")
 
     def launch(self, inputs=None, outputs=None, stream_ptr=None):
         """Enqueue one replay and return its number.  None keeps the previous

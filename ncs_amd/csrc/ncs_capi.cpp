@@ -1,6 +1,7 @@
 // ncs_capi.cpp -- C ABI of libncs_hip.so: context, OTF tables, launches,
 // host/device frame batches, the C-library drop-in, graphs and autotuning.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -20,6 +21,9 @@ hipError_t launch_adu_to_photons(const uint16_t *adu, const float *offset, const
                                  float *out, long long n, hipStream_t s);
 hipError_t launch_gamma_from_camera(const float *var, const float *gain, float *gamma,
                                     long long n, hipStream_t s);
+hipError_t launch_adu_levels(const uint16_t *adu, const float *offset, const float *gains,
+                             const float *gammas, long long stride, const int *t, float *out,
+                             float *gamma, long long n, hipStream_t s);
 hipError_t net_pack(const float *P, void *packs, hipStream_t s);
 size_t net_packs_bytes();
 hipError_t net_forward(const float *x, float *y, void *acts, const void *packs, const float *P,
@@ -930,6 +934,11 @@ struct ncs_pipeline {
   bool pad = false, general = false, keep = false, learned = false;
   Schedule sched;
   DevBuf raw, data, gamma, out, offset, gain, frames, queue, results, u0;
+  // several gain amplifiers: per frame, the number of levels (1 = one gain
+  // map) and the division points as integers; level maps [levels][pix]
+  std::vector<int> levels;
+  std::vector<std::array<int, NCS_MAX_GAIN_LEVELS - 1>> level_points;
+  DevBuf level_gains, level_gammas;
   char *h_res[kReplaySlots] = {};  // per slot: iterations, status, cost
   std::vector<char> gamma_set;
   hipStream_t stream = nullptr;
@@ -949,7 +958,8 @@ extern "C" void ncs_hip_pipeline_destroy(ncs_pipeline *pl) {
     (void)hipEventSynchronize(pl->done[(pl->launched - 1) % kReplaySlots]);
   if (pl->exec) (void)hipGraphExecDestroy(pl->exec);
   DevBuf *bufs[] = {&pl->raw, &pl->data, &pl->gamma, &pl->out, &pl->offset, &pl->gain,
-                    &pl->frames, &pl->queue, &pl->results, &pl->u0};
+                    &pl->frames, &pl->queue, &pl->results, &pl->u0,
+                    &pl->level_gains, &pl->level_gammas};
   for (DevBuf *b : bufs) b->release();
   for (int k = 0; k < kReplaySlots; k++) {
     if (pl->h_res[k]) (void)hipHostFree(pl->h_res[k]);
@@ -1041,6 +1051,8 @@ extern "C" int ncs_hip_pipeline_create(ncs_ctx *ctx, const ncs_pipeline_frame *f
   for (int k = 0; k < kReplaySlots; k++)
     HIPCHK(hipEventCreateWithFlags(&pl->done[k], hipEventDisableTiming));
   pl->gamma_set.assign(n_frames, 0);
+  pl->levels.assign(n_frames, 1);
+  pl->level_points.resize(n_frames);
   pl->sched = resolve_schedule(ctx, params, pl->pad);
   *out = pl.release();
   return NCS_OK;
@@ -1059,8 +1071,17 @@ static int pipeline_idle(ncs_pipeline *pl) {
   return pl->launched ? wait_replay(pl, pl->launched - 1) : NCS_OK;
 }
 
+// The next launch captures the graph again (the pipeline is idle).
+static void drop_graph(ncs_pipeline *pl) {
+  if (pl->exec) (void)hipGraphExecDestroy(pl->exec);
+  pl->exec = nullptr;
+}
+
 extern "C" int ncs_hip_pipeline_set_gamma(ncs_pipeline *pl, int frame, const float *gamma) {
   if (!pl || !gamma || frame < 0 || frame >= pl->n) return fail(NCS_E_ARG, "bad arguments");
+  if (pl->levels[frame] > 1)
+    return fail(NCS_E_ARG, "gamma of frame " + std::to_string(frame) +
+                               " comes from its gain levels at every replay");
   HIPCHK(set_device(pl->ctx));
   int rc = pipeline_idle(pl);
   if (rc) return rc;
@@ -1089,6 +1110,60 @@ extern "C" int ncs_hip_pipeline_set_camera(ncs_pipeline *pl, int frame, const fl
                                   pl->gamma.as<float>() + o, (long long)n, pl->stream));
   HIPCHK(hipStreamSynchronize(pl->stream));
   pl->gamma_set[frame] = 1;
+  if (pl->levels[frame] != 1) {
+    pl->levels[frame] = 1;
+    drop_graph(pl);
+  }
+  return NCS_OK;
+}
+
+extern "C" int ncs_hip_pipeline_set_camera_levels(ncs_pipeline *pl, int frame, int n_levels,
+                                                  const double *points, const float *offset,
+                                                  const float *gains, const float *variances) {
+  if (!pl || !offset || !gains || !variances || frame < 0 || frame >= pl->n)
+    return fail(NCS_E_ARG, "bad arguments");
+  if (pl->input != NCS_INPUT_ADU_U16)
+    return fail(NCS_E_ARG, "camera maps belong to NCS_INPUT_ADU_U16 pipelines");
+  if (n_levels < 1 || n_levels > NCS_MAX_GAIN_LEVELS)
+    return fail(NCS_E_ARG, "n_levels must be 1.." + std::to_string(NCS_MAX_GAIN_LEVELS));
+  if (n_levels > 1 && !points) return fail(NCS_E_ARG, "division points are NULL");
+  // ADU is an integer, so ADU >= p exactly when ADU >= ceil(p)
+  std::array<int, NCS_MAX_GAIN_LEVELS - 1> t;
+  t.fill(65536);
+  for (int j = 0; j + 1 < n_levels; j++) {
+    const double p = points[j];
+    if (std::isnan(p)) return fail(NCS_E_ARG, "a division point is NaN");
+    if (j > 0 && p < points[j - 1]) return fail(NCS_E_ARG, "division points must ascend");
+    t[j] = (int)std::min(65536.0, std::max(0.0, std::ceil(p)));
+  }
+  HIPCHK(set_device(pl->ctx));
+  int rc = pipeline_idle(pl);
+  if (rc) return rc;
+  if (n_levels == 1)
+    return ncs_hip_pipeline_set_camera(pl, frame, offset, gains, variances);
+  const size_t n = frame_pixels(pl->shape[frame]);
+  const size_t o = pl->off[frame];
+  const size_t lb = (size_t)NCS_MAX_GAIN_LEVELS * pl->pix * sizeof(float);
+  HIPCHK(pl->level_gains.ensure(lb));
+  HIPCHK(pl->level_gammas.ensure(lb));
+  HIPCHK(hipMemcpy(pl->offset.as<float>() + o, offset, n * 4, hipMemcpyHostToDevice));
+  for (int l = 0; l < n_levels; l++) {
+    float *g = pl->level_gains.as<float>() + (size_t)l * pl->pix + o;
+    HIPCHK(hipMemcpy(g, gains + (size_t)l * n, n * 4, hipMemcpyHostToDevice));
+    // the variance goes through the output buffer, which every replay rewrites
+    HIPCHK(hipMemcpy(pl->out.as<float>() + o, variances + (size_t)l * n, n * 4,
+                     hipMemcpyHostToDevice));
+    HIPCHK(launch_gamma_from_camera(pl->out.as<float>() + o, g,
+                                    pl->level_gammas.as<float>() + (size_t)l * pl->pix + o,
+                                    (long long)n, pl->stream));
+    HIPCHK(hipStreamSynchronize(pl->stream));
+  }
+  pl->gamma_set[frame] = 1;
+  if (pl->levels[frame] != n_levels || pl->level_points[frame] != t) {
+    pl->levels[frame] = n_levels;
+    pl->level_points[frame] = t;
+    drop_graph(pl);  // the division points are arguments of the captured kernel
+  }
   return NCS_OK;
 }
 
@@ -1109,9 +1184,15 @@ static int capture_replay(ncs_pipeline *pl, hipGraph_t *out) {
   if (pl->input == NCS_INPUT_ADU_U16) {
     for (int i = 0; i < pl->n && e == hipSuccess; i++) {
       const size_t o = pl->off[i];
-      e = launch_adu_to_photons(pl->raw.as<uint16_t>() + o, pl->offset.as<float>() + o,
-                                pl->gain.as<float>() + o, pl->data.as<float>() + o,
-                                (long long)frame_pixels(pl->shape[i]), s);
+      const long long n = (long long)frame_pixels(pl->shape[i]);
+      if (pl->levels[i] > 1)
+        e = launch_adu_levels(pl->raw.as<uint16_t>() + o, pl->offset.as<float>() + o,
+                              pl->level_gains.as<float>() + o, pl->level_gammas.as<float>() + o,
+                              (long long)pl->pix, pl->level_points[i].data(),
+                              pl->data.as<float>() + o, pl->gamma.as<float>() + o, n, s);
+      else
+        e = launch_adu_to_photons(pl->raw.as<uint16_t>() + o, pl->offset.as<float>() + o,
+                                  pl->gain.as<float>() + o, pl->data.as<float>() + o, n, s);
     }
   }
   const float *u0 = nullptr;

@@ -180,3 +180,26 @@ def test_block_geometry_ragged_edges_match_masked_oracle():
             hh, ww = min(16, H - by), min(16, W - bx)
             assert numpy.max(numpy.abs(out[0][by:by + hh, bx:bx + ww] - ref["u"][k][:hh, :ww])) < 0.05
             k += 1
+
+
+def test_pyncs_r14_segmentation_is_the_reference_geometry():
+    """pyNCS splits a square frame into R1 x R1 blocks with a one-pixel halo
+    (python3-6/pyNCS/denoisetools.py:27-86).  With R1 = 14 its windows are
+    the reference geometry's 16 x 16 windows, so solving pyNCS's windows one
+    by one and stitching their interiors gives the frame path's image bit
+    for bit."""
+    from oracle import camera
+    _say("pyNCS R1 = 14 segmentation, 112 x 112")
+    fr = sim.make_frame(71, (112, 112))
+    ctx = _lib.context(0)
+    ctx.set_otf_mask(masks.to_fft_order(masks.otf_mask()).astype(numpy.float32))
+    p = _lib.default_params()
+    d = camera.segpadimg(fr["data"].astype(numpy.float32), 14).astype(numpy.float32)
+    g = camera.segpadimg(fr["gamma"].astype(numpy.float32), 14).astype(numpy.float32)
+    u, it, st, co = ctx.solve_tiles(d, g, 0.1, p)
+    out = numpy.zeros((112, 112), numpy.float32)
+    ctx.reduce_frames([fr["data"].astype(numpy.float32)], [fr["gamma"].astype(numpy.float32)],
+                      [out], [0.1], _lib.GEOM_REFERENCE, p)
+    seg = camera.stitchpadimg(u)
+    _say("max |diff|", float(numpy.max(numpy.abs(seg - out))), "status", numpy.unique(st))
+    assert numpy.array_equal(seg.astype(numpy.float32), out)

@@ -127,6 +127,70 @@ def test_pipeline_raw_adu_input_converts_on_the_device(otf):
         pl.close()
 
 
+def test_pipeline_gain_levels_pick_maps_by_adu(otf):
+    """Three gain amplifiers (matlab/genmaps.m:29-62): each pixel's gain and
+    variance maps are picked by its ADU against the division points, ties
+    included, and non-integer points round as ADU >= point does.  alpha = 0
+    returns the photon image, bit for bit the oracle's genmaps + conversion;
+    alpha = 0.1 gives the eager batch's image and tile results on those
+    photons and gammas.  New points recapture the graph; one level again is
+    the single-map conversion."""
+    from oracle import camera
+    ctx = otf
+    pin = _Pinned()
+    shape = (77, 130)
+    f = sim.make_frame(42, shape)
+    adu = numpy.clip(numpy.rint(f["adu"]), 0, 65535).astype(numpy.uint16)
+    off = f["offset"].astype(numpy.float32)
+    gains = numpy.stack([f["gain"], 1.3 * f["gain"], 2.1 * f["gain"]]).astype(numpy.float32)
+    var = numpy.stack([f["var"], 4.0 * f["var"], 9.0 * f["var"]]).astype(numpy.float32)
+    lo, hi = (int(v) for v in numpy.percentile(adu, [40, 80]))
+    adu[0, :4] = [lo - 1, lo, hi - 1, hi]  # ties at the division points
+    src = pin(shape, numpy.uint16)
+    src[...] = adu
+    out = pin(shape, numpy.float32)
+    p = _lib.default_params()
+
+    def expect(points):
+        gs, vs = camera.genmaps(adu[None], gains, var, points)
+        return (camera.adu_to_photons(adu, off, gs[0]),
+                camera.gamma_from_camera(vs[0], gs[0]))
+
+    for alpha in (0.0, 0.1):
+        pl = _lib.Pipeline(ctx, [shape], [alpha], params=p, input_format=_lib.INPUT_ADU_U16,
+                           keep_tile_results=True)
+        for points in ([lo, hi], [lo - 0.5, hi + 0.25]):
+            pl.set_camera_levels(0, points, off, gains, var)
+            ph, gm = expect(points)
+            pl.launch([src], [out])
+            pl.wait()
+            ref = _eager(ctx, [ph], [gm], [alpha], p)
+            _say("levels", points, "alpha", alpha, "max |diff|",
+                 float(numpy.max(numpy.abs(out - ref[0][0]))))
+            if alpha == 0.0:
+                assert numpy.array_equal(out, ph)
+            assert numpy.array_equal(out, ref[0][0])
+            assert numpy.array_equal(pl.tile_results()[2], ref[3])
+        with pytest.raises(_lib.NCSHipError):
+            pl.set_gamma(0, gm)  # gamma comes from the levels
+        pl.set_camera(0, off, gains[1], var[1])
+        pl.launch([src], [out])
+        pl.wait()
+        ph1 = camera.adu_to_photons(adu, off, gains[1])
+        ref = _eager(ctx, [ph1], [camera.gamma_from_camera(var[1], gains[1])], [alpha], p)
+        assert numpy.array_equal(out, ref[0][0])
+        pl.close()
+    pl = _lib.Pipeline(ctx, [shape], [0.1], params=p, input_format=_lib.INPUT_ADU_U16)
+    with pytest.raises(_lib.NCSHipError):
+        pl.set_camera_levels(0, [hi, lo], off, gains, var)  # descending
+    with pytest.raises(_lib.NCSHipError):
+        five = numpy.concatenate([gains, gains[:2]])
+        pl.set_camera_levels(0, [1, 2, 3, 4], off, five, numpy.concatenate([var, var[:2]]))
+    with pytest.raises(ValueError):
+        pl.set_camera_levels(0, [lo], off, gains, var)  # 3 levels take 2 points
+    pl.close()
+
+
 def test_pipeline_blocks_geometry_uneven_frames(otf):
     ctx = otf
     pin = _Pinned()
