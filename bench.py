@@ -576,8 +576,27 @@ def run_stream(args, weights, lin_coef=None):
         raise SystemExit("--stream-depth >= 1 and 0 <= --stream-lag < depth * %d"
                          % _lib.PIPELINE_REPLAYS)
     n_out = lag + 2  # an output buffer is reused only after its frame was consumed
+    # each start's schedule is autotuned on the first frame's photons, as the
+    # batch workloads tune theirs
+    ph0 = (adu[0].astype(numpy.float32) - off32) / gain32
+    ph0[ph0 <= 0] = numpy.float32(1e-6)
+    td = torch.from_numpy(ph0).to(dev)
+    tg = torch.from_numpy(var32 / (gain32 * gain32)).to(dev)
+    tout = torch.zeros_like(td)
+    desc0 = [dict(data=td.data_ptr(), gamma=tg.data_ptr(), out=tout.data_ptr(),
+                  height=shape[0], width=shape[1], alpha=ALPHA, geometry=_lib.GEOM_REFERENCE)]
+    tunes = {}
     for label, start in modes:
-        p = _lib.default_params(max_iterations=MAX_IT, start=start)
+        if start == _lib.START_LEARNED:
+            p = _lib.default_params(max_iterations=MAX_IT, start=start)
+        else:
+            tune = ctx.autotune(desc0, _lib.default_params(max_iterations=MAX_IT, start=start),
+                                tune_history=True)
+            tunes[label] = tune
+            print("[bench] stream %s autotune: %s" % (label, tune), file=sys.stderr, flush=True)
+            p = _lib.default_params(max_iterations=MAX_IT, start=start, history=tune["history"],
+                                    waves_per_cu=tune["waves_per_cu"],
+                                    waves_per_group=tune["waves_per_group"])
         pls = [_lib.Pipeline(ctx, [shape], [ALPHA], params=p, input_format=_lib.INPUT_ADU_U16,
                              keep_tile_results=True) for _ in range(depth)]
         for pl in pls:
@@ -613,14 +632,15 @@ def run_stream(args, weights, lin_coef=None):
         wall = time.perf_counter() - t0
         for pl in pls:
             pl.close()
-        results[label] = dict(wall=wall, lat=lat, kept=kept, costs=costs)
+        results[label] = dict(wall=wall, lat=lat, kept=kept, costs=costs, params=p.as_dict())
         print("[bench] stream %s: %.1f Mpix/s, latency p50 %.2f ms p99 %.2f ms"
               % (label, n * shape[0] * shape[1] / wall / 1e6, numpy.median(lat) * 1e3,
                  numpy.percentile(lat, 99) * 1e3), file=sys.stderr, flush=True)
     # parity: the checked frames against the eager plain batch on the same
     # photons (host conversion in float32)
     par = {}
-    p0 = _lib.default_params(max_iterations=MAX_IT)
+    # the eager batch with the plain mode's schedule: bit-exact for that mode
+    p0 = _lib.default_params(**results["plain"]["params"])
     for k in check_frames:
         ph = (adu[k].astype(numpy.float32) - off32) / gain32
         ph[ph <= 0] = numpy.float32(1e-6)
@@ -647,6 +667,8 @@ def run_stream(args, weights, lin_coef=None):
         return {"value": round(pix / r["wall"] / 1e6, 2), "unit": "Mpix/s",
                 "latency_ms_p50": round(float(numpy.median(r["lat"])) * 1e3, 3),
                 "latency_ms_p99": round(float(numpy.percentile(r["lat"], 99)) * 1e3, 3),
+                "schedule": {k: r["params"][k] for k in ("history", "waves_per_cu",
+                                                         "waves_per_group")},
                 "parity_vs_eager_plain": par[label]}
 
     line = {
@@ -659,8 +681,8 @@ def run_stream(args, weights, lin_coef=None):
                 "segment, background 10->40 photons/px, fixed sCMOS maps, uint16 ADU",
         "config": {"workload": "%d frames of 2304x2304 uint16 ADU streamed frame by frame "
                                "through %d captured pipelines (H2D, ADU->photons, %s start, "
-                               "solve+stitch, D2H), host %d frames ahead, alpha %.1f, m=5, "
-                               "eps 1e-5" % (n, depth, best, lag, ALPHA),
+                               "solve+stitch, D2H), host %d frames ahead, alpha %.1f, "
+                               "autotuned schedule, eps 1e-5" % (n, depth, best, lag, ALPHA),
                    "frames": n, "height": shape[0], "width": shape[1], "start": best,
                    "pipelines": depth, "frames_ahead": lag},
         "latency_ms_p50": summary(best)["latency_ms_p50"],
