@@ -115,8 +115,22 @@ int ncs_hip_device_info(ncs_ctx *ctx, char *buf, int buflen);
  * passes the reference's checkOTFMask. */
 int ncs_hip_set_otf_mask(ncs_ctx *ctx, const float *mask_fft_order);
 
-/* Number of tiles a frame produces with the given geometry. */
+/* An r x r mask (FFT order, r even, 4..16) for sub-regions of side r, the
+ * clib's r_size (ncs.c:41-147 takes any even r; pyCNCS derives it from the
+ * mask, ncs_c.py:250-276).  Frames then use the reference geometry with
+ * windows of side r, stride r - 2, each solved in the top-left of a 16 x 16
+ * tile whose other pixels are left out of the cost; only the plain start is
+ * accepted, and the tile and network entry points, which take 16 x 16
+ * tiles, are refused.  r = 16 is ncs_hip_set_otf_mask. */
+int ncs_hip_set_otf_mask_window(ncs_ctx *ctx, const float *mask_fft_order, int r);
+
+/* Number of tiles a frame produces with the given geometry (16 x 16
+ * windows); -1 if the frame is too small.  Per-tile result arrays hold
+ * this many entries. */
 long long ncs_hip_tile_count(int height, int width, int geometry);
+/* The same for windows of side r (the reference geometry only for r < 16):
+ * the count after ncs_hip_set_otf_mask_window(ctx, mask, r). */
+long long ncs_hip_tile_count_window(int height, int width, int geometry, int r);
 
 /* Whole frames, host memory in and out; synchronous.  Replaces the batched
  * host reduceNoise (pyOpenCLNCS/ncs.py:47-198): every frame may have its own

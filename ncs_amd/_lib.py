@@ -112,7 +112,9 @@ SIGNATURES = {
     "ncs_hip_last_error": (ctypes.c_char_p, []),
     "ncs_hip_device_info": (_i, [_vp, ctypes.c_char_p, _i]),
     "ncs_hip_set_otf_mask": (_i, [_vp, _vp]),
+    "ncs_hip_set_otf_mask_window": (_i, [_vp, _vp, _i]),
     "ncs_hip_tile_count": (_ll, [_i, _i, _i]),
+    "ncs_hip_tile_count_window": (_ll, [_i, _i, _i, _i]),
     "ncs_hip_reduce_frames": (_i, [_vp, _F, _i, _P, _vp, _vp, _vp]),
     "ncs_hip_reduce_frames_device": (_i, [_vp, _F, _i, _P, _vp, _vp, _vp, _vp, _vp]),
     "ncs_hip_solve_tiles": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _f, _P]),
@@ -212,8 +214,11 @@ def kernel_semantics_params(**overrides):
     return p
 
 
-def tile_count(height, width, geometry=GEOM_REFERENCE):
-    return int(load().ncs_hip_tile_count(int(height), int(width), int(geometry)))
+def tile_count(height, width, geometry=GEOM_REFERENCE, window=16):
+    """Tiles of a frame; window is the side of the reference geometry's
+    windows (the mask's size), -1 when the frame is too small."""
+    return int(load().ncs_hip_tile_count_window(int(height), int(width), int(geometry),
+                                                int(window)))
 
 
 def _ptr(a):
@@ -248,11 +253,16 @@ class Context(object):
         return buf.value.decode()
 
     def set_otf_mask(self, mask_fft_order):
+        """FFT-order mask, 16 x 16, or r x r (r even, 4..16) for windows of
+        side r with the plain start and the reference geometry."""
         m = numpy.ascontiguousarray(mask_fft_order, dtype=numpy.float32)
-        if m.shape != (16, 16):
-            raise ValueError("the OTF mask must be 16 x 16")
-        check(self.lib.ncs_hip_set_otf_mask(self.handle, _ptr(m)), "ncs_hip_set_otf_mask")
+        r = m.shape[0] if m.ndim == 2 else 0
+        if m.shape != (r, r) or r < 4 or r > 16 or r % 2:
+            raise ValueError("the OTF mask must be r x r with r even, 4..16")
+        check(self.lib.ncs_hip_set_otf_mask_window(self.handle, _ptr(m), r),
+              "ncs_hip_set_otf_mask_window")
         self.mask_fft = m.copy()
+        self.window = r
 
     def set_linear_start(self, coef):
         """Coefficients [A (16x16), B (16x16), c] of the learned linear start."""
@@ -283,7 +293,7 @@ class Context(object):
             fr.data_stride = fr.gamma_stride = fr.out_stride = 0
             fr.alpha = float(alphas[i])
             fr.geometry = int(geometry[i] if hasattr(geometry, "__len__") else geometry)
-            nt = tile_count(fr.height, fr.width, fr.geometry)
+            nt = tile_count(fr.height, fr.width, fr.geometry, getattr(self, "window", 16))
             if nt < 0:
                 raise ValueError("frame %d of shape %s is too small for this tiling"
                                  % (i, d.shape))

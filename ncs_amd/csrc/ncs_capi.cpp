@@ -100,6 +100,7 @@ struct ncs_ctx {
   DevBuf tables, frames, queue, data, gamma, out, iters, status, cost, tiles_u;
   bool has_mask = false;
   bool general = false;
+  int win = 16;  // window side of the reference geometry (the mask's size)
   float mask_fft[256];
   // host copy of the per-lane tables (mask part and linear-start part)
   std::vector<float> tab_host = std::vector<float>(kTableFloats, 0.0f);
@@ -190,12 +191,19 @@ extern "C" int ncs_hip_device_info(ncs_ctx *ctx, char *buf, int buflen) {
 
 // Orthonormal real DFT basis of length 16: rows cos k (k = 0..8) then
 // sin k (k = 1..7).
-static double rbasis(int a, int i) {
+// Orthonormal real DFT basis of length r (even, <= 16) in the 16 slots of
+// the tile layout: slot k holds cos k (k = 0..r/2), slot 8 + k holds sin k
+// (k = 1..r/2 - 1), so a cosine row and its sine partner are always 8 slots
+// apart; the other slots, and pixels i >= r, are zero.
+static bool slot_used(int a, int r) { return a <= 8 ? a <= r / 2 : a - 8 < r / 2; }
+static double rbasis(int a, int i, int r = 16) {
   const double PI = 3.14159265358979323846;
-  if (a == 0) return 0.25;
-  if (a == 8) return (i & 1) ? -0.25 : 0.25;
-  if (a < 8) return std::sqrt(2.0) / 4.0 * std::cos(2.0 * PI * a * i / 16.0);
-  return std::sqrt(2.0) / 4.0 * std::sin(2.0 * PI * (a - 8) * i / 16.0);
+  if (i >= r || !slot_used(a, r)) return 0.0;
+  const double n0 = 1.0 / std::sqrt((double)r), n1 = std::sqrt(2.0 / r);
+  if (a == 0) return n0;
+  if (a == r / 2) return (i & 1) ? -n0 : n0;
+  if (a <= 8) return n1 * std::cos(2.0 * PI * a * i / r);
+  return n1 * std::sin(2.0 * PI * (a - 8) * i / r);
 }
 
 static int freq_of(int a) { return a <= 8 ? a : a - 8; }
@@ -209,18 +217,23 @@ static void split_f16(double v, uint16_t &hi, uint16_t &lo) {
 }
 
 extern "C" int ncs_hip_set_otf_mask(ncs_ctx *ctx, const float *mask) {
+  return ncs_hip_set_otf_mask_window(ctx, mask, 16);
+}
+
+extern "C" int ncs_hip_set_otf_mask_window(ncs_ctx *ctx, const float *mask, int win) {
   if (!ctx || !mask) return fail(NCS_E_ARG, "bad arguments");
-  double q[16][16], qcs[16][16];
+  if (win < 4 || win > 16 || (win & 1)) return fail(NCS_E_ARG, "window side must be even, 4..16");
+  double q[16][16] = {}, qcs[16][16] = {};
   double qmax = 0.0;
-  for (int k = 0; k < 16; k++)
-    for (int l = 0; l < 16; l++) {
-      double m = mask[k * 16 + l];
+  for (int k = 0; k < win; k++)
+    for (int l = 0; l < win; l++) {
+      double m = mask[k * win + l];
       if (!std::isfinite(m)) return fail(NCS_E_MASK, "mask has non-finite values");
       q[k][l] = m * m;
     }
-  for (int k = 0; k < 16; k++)
-    for (int l = 0; l < 16; l++) {
-      qcs[k][l] = 0.5 * (q[k][l] + q[(16 - k) & 15][(16 - l) & 15]);
+  for (int k = 0; k < win; k++)
+    for (int l = 0; l < win; l++) {
+      qcs[k][l] = 0.5 * (q[k][l] + q[(win - k) % win][(win - l) % win]);
       qmax = std::max(qmax, std::fabs(qcs[k][l]));
     }
   std::vector<float> &tab = ctx->tab_host;
@@ -229,12 +242,12 @@ extern "C" int ncs_hip_set_otf_mask(ncs_ctx *ctx, const float *mask) {
     int g = lane >> 4, x = lane & 15;
     for (int r = 0; r < 4; r++) {
       int b = 4 * g + r, a = x;
-      tab[lane * kTableLaneFloats + r] = (float)rbasis(x, 4 * g + r);
-      tab[lane * kTableLaneFloats + 4 + r] = (float)rbasis(4 * g + r, x);
+      tab[lane * kTableLaneFloats + r] = (float)rbasis(x, 4 * g + r, win);
+      tab[lane * kTableLaneFloats + 4 + r] = (float)rbasis(4 * g + r, x, win);
       // FP16 hi/lo pairs of the same entries, each duplicated into both
       // halves of an 8-wide FP16 operand (slots r and 4 + r)
       for (int t = 0; t < 2; t++) {
-        const double v = t == 0 ? rbasis(x, 4 * g + r) : rbasis(4 * g + r, x);
+        const double v = t == 0 ? rbasis(x, 4 * g + r, win) : rbasis(4 * g + r, x, win);
         uint16_t hi, lo;
         split_f16(v, hi, lo);
         uint16_t *h = reinterpret_cast<uint16_t *>(&tab[lane * kTableLaneFloats + 24 + 8 * t]);
@@ -242,8 +255,9 @@ extern "C" int ncs_hip_set_otf_mask(ncs_ctx *ctx, const float *mask) {
         h[8 + r] = h[12 + r] = lo;  // entry 7 or 9: lo
       }
       int k = freq_of(a), l = freq_of(b);
-      double qee = 0.5 * (qcs[k][l] + qcs[(16 - k) & 15][l]);
-      double qoo = 0.5 * (qcs[k][l] - qcs[(16 - k) & 15][l]);
+      const bool used = slot_used(a, win) && slot_used(b, win);
+      double qee = used ? 0.5 * (qcs[k][l] + qcs[(win - k) % win][l]) : 0.0;
+      double qoo = used ? 0.5 * (qcs[k][l] - qcs[(win - k) % win][l]) : 0.0;
       double sgn = ((a <= 8) == (b <= 8)) ? -1.0 : 1.0;
       tab[lane * kTableLaneFloats + 8 + r] = (float)(2.0 * qee);
       tab[lane * kTableLaneFloats + 12 + r] = (float)(sgn * 2.0 * qoo);
@@ -255,7 +269,9 @@ extern "C" int ncs_hip_set_otf_mask(ncs_ctx *ctx, const float *mask) {
   HIPCHK(hipMemcpy(ctx->tables.p, tab.data(), tab.size() * sizeof(float),
                    hipMemcpyHostToDevice));
   ctx->general = qo_max > 1e-12 * std::max(qmax, 1e-300);
-  std::memcpy(ctx->mask_fft, mask, sizeof(ctx->mask_fft));
+  std::memset(ctx->mask_fft, 0, sizeof(ctx->mask_fft));
+  std::memcpy(ctx->mask_fft, mask, (size_t)win * win * sizeof(float));
+  ctx->win = win;
   ctx->has_mask = true;
   return NCS_OK;
 }
@@ -287,20 +303,39 @@ extern "C" int ncs_hip_set_linear_start(ncs_ctx *ctx, const float *coef) {
 // tiling
 // ---------------------------------------------------------------------------
 
-static long long tiles_axis(int n, int geom) {
+// Windows along an axis of n pixels: the reference geometry with side r
+// steps by s = r - 2 and needs n >= s; 1 + ceil((n - s) / s) windows
+// (ncs.c:71-143).
+static long long tiles_axis(int n, int geom, int r = 16) {
   if (geom == NCS_GEOM_REFERENCE) {
-    if (n < 14) return -1;
-    return 1 + (n - 14 + 13) / 14;  // 1 + ceil((n - 14) / 14), ncs.c:71-143
+    const int s = r - 2;
+    if (n < s) return -1;
+    return 1 + (n - s + s - 1) / s;
   }
   if (n < 1) return -1;
   return (n + 15) / 16;
 }
 
-extern "C" long long ncs_hip_tile_count(int height, int width, int geometry) {
+extern "C" long long ncs_hip_tile_count_window(int height, int width, int geometry, int r) {
   if (geometry != NCS_GEOM_REFERENCE && geometry != NCS_GEOM_BLOCKS) return -1;
-  long long a = tiles_axis(height, geometry), b = tiles_axis(width, geometry);
+  if (r < 4 || r > 16 || (r & 1) || (geometry == NCS_GEOM_BLOCKS && r != 16)) return -1;
+  long long a = tiles_axis(height, geometry, r), b = tiles_axis(width, geometry, r);
   if (a < 0 || b < 0) return -1;
   return a * b;
+}
+
+extern "C" long long ncs_hip_tile_count(int height, int width, int geometry) {
+  return ncs_hip_tile_count_window(height, width, geometry, 16);
+}
+
+// A window side other than 16 (a smaller mask) takes the reference
+// geometry and the plain start: the block geometry, tile entry points and
+// the learned starts are built on 16 x 16 tiles.
+static int check_window(const ncs_ctx *ctx, const ncs_params *p) {
+  if (ctx->win != 16 && p->start != NCS_START_PLAIN)
+    return fail(NCS_E_ARG, "a " + std::to_string(ctx->win) + " x " + std::to_string(ctx->win) +
+                               " mask takes the plain start");
+  return NCS_OK;
 }
 
 static int check_params(const ncs_params *p) {
@@ -332,17 +367,21 @@ static int check_params(const ncs_params *p) {
 }
 
 // Builds kernel frame descriptors (device pointers) and the tile total.
-static int build_frames(const ncs_frame *frames, int n, std::vector<FrameDev> &fd,
-                        long long &total) {
+static int build_frames(const ncs_ctx *ctx, const ncs_frame *frames, int n,
+                        std::vector<FrameDev> &fd, long long &total) {
+  const int win = ctx->win;
   fd.resize(n);
   total = 0;
   for (int i = 0; i < n; i++) {
     const ncs_frame &f = frames[i];
     if (!f.data || !f.gamma) return fail(NCS_E_ARG, "frame without data or gamma");
-    long long nt = ncs_hip_tile_count(f.height, f.width, f.geometry);
+    if (win != 16 && f.geometry != NCS_GEOM_REFERENCE)
+      return fail(NCS_E_ARG, "frame " + std::to_string(i) + ": a " + std::to_string(win) +
+                  " x " + std::to_string(win) + " mask takes the reference geometry");
+    long long nt = ncs_hip_tile_count_window(f.height, f.width, f.geometry, win);
     if (nt < 0)
       return fail(NCS_E_ARG, "frame " + std::to_string(i) + ": bad size or geometry (the "
-                  "reference tiling needs height, width >= 14)");
+                  "reference tiling needs height, width >= " + std::to_string(win - 2) + ")");
     if (!std::isfinite(f.alpha)) return fail(NCS_E_ARG, "non-finite alpha");
     FrameDev &d = fd[i];
     d.data = f.data;
@@ -359,8 +398,8 @@ static int build_frames(const ncs_frame *frames, int n, std::vector<FrameDev> &f
       return fail(NCS_E_ARG, "frame " + std::to_string(i) + " spans 2^31 or more elements");
     d.alpha = f.alpha;
     d.geom = f.geometry;
-    d.ty = (int)tiles_axis(f.width, f.geometry);
-    d.pad = 0;
+    d.ty = (int)tiles_axis(f.width, f.geometry, win);
+    d.win = win;
     d.first = total;
     total += nt;
   }
@@ -399,7 +438,8 @@ static int upload_frames(ncs_ctx *ctx, const std::vector<FrameDev> &fd, hipStrea
   return NCS_OK;
 }
 
-static bool any_pad(const ncs_frame *frames, int n) {
+static bool any_pad(const ncs_ctx *ctx, const ncs_frame *frames, int n) {
+  if (ctx->win != 16) return true;  // pixels outside the window are padding
   for (int i = 0; i < n; i++)
     if (frames[i].geometry != NCS_GEOM_REFERENCE) return true;
   return false;
@@ -490,6 +530,8 @@ extern "C" int ncs_hip_reduce_frames_device(ncs_ctx *ctx, const ncs_frame *frame
   if (!ctx || !frames || n_frames <= 0) return fail(NCS_E_ARG, "bad arguments");
   int rc = check_params(params);
   if (rc) return rc;
+  rc = check_window(ctx, params);
+  if (rc) return rc;
   if (params->start == NCS_START_TILES && !u0_tiles)
     return fail(NCS_E_ARG, "NCS_START_TILES needs u0_tiles");
   if (params->start == NCS_START_LEARNED && !ctx->has_net)
@@ -498,7 +540,7 @@ extern "C" int ncs_hip_reduce_frames_device(ncs_ctx *ctx, const ncs_frame *frame
     return fail(NCS_E_NOWEIGHTS, "linear start requested but no coefficients are set");
   std::vector<FrameDev> fd;
   long long total = 0;
-  rc = build_frames(frames, n_frames, fd, total);
+  rc = build_frames(ctx, frames, n_frames, fd, total);
   if (rc) return rc;
   HIPCHK(set_device(ctx));
   hipStream_t s = (hipStream_t)stream;  // 0 = the default (NULL) stream
@@ -512,7 +554,7 @@ extern "C" int ncs_hip_reduce_frames_device(ncs_ctx *ctx, const ncs_frame *frame
                     ctx->net_packs.p, ctx->net_params.as<float>(), grid, s));
     u0 = ctx->u0_buf.as<float>();
   }
-  return enqueue_solve(ctx, ctx->frames.as<FrameDev>(), n_frames, any_pad(frames, n_frames),
+  return enqueue_solve(ctx, ctx->frames.as<FrameDev>(), n_frames, any_pad(ctx, frames, n_frames),
                        total, params, u0, nullptr,
                        iterations, status, cost, ctx->queue.as<unsigned int>(), s);
 }
@@ -525,6 +567,8 @@ extern "C" int ncs_hip_solve_tiles_device(ncs_ctx *ctx, const float *data_in,
   if (!ctx || !data_in || !gamma || num_tiles < 0) return fail(NCS_E_ARG, "bad arguments");
   if (num_tiles == 0) return NCS_OK;
   int rc = check_params(params);
+  if (rc) return rc;
+  rc = check_window(ctx, params);
   if (rc) return rc;
   ncs_frame f;
   f.data = data_in;
@@ -552,13 +596,15 @@ extern "C" int ncs_hip_reduce_frames(ncs_ctx *ctx, const ncs_frame *frames, int 
   if (!ctx || !frames || n_frames <= 0) return fail(NCS_E_ARG, "bad arguments");
   int rc = check_params(params);
   if (rc) return rc;
+  rc = check_window(ctx, params);
+  if (rc) return rc;
   if (params->start != NCS_START_PLAIN && params->start != NCS_START_LINEAR)
     return fail(NCS_E_ARG, "host-memory batches use the plain or the linear start");
   if (params->start == NCS_START_LINEAR && !ctx->has_linear)
     return fail(NCS_E_NOWEIGHTS, "linear start requested but no coefficients are set");
   std::vector<FrameDev> fd;
   long long total = 0;
-  rc = build_frames(frames, n_frames, fd, total);
+  rc = build_frames(ctx, frames, n_frames, fd, total);
   if (rc) return rc;
   HIPCHK(set_device(ctx));
   hipStream_t s = ctx->stream;
@@ -601,7 +647,7 @@ extern "C" int ncs_hip_reduce_frames(ncs_ctx *ctx, const ncs_frame *frames, int 
   }
   rc = upload_frames(ctx, fd, s);
   if (rc) return rc;
-  rc = enqueue_solve(ctx, ctx->frames.as<FrameDev>(), n_frames, any_pad(frames, n_frames),
+  rc = enqueue_solve(ctx, ctx->frames.as<FrameDev>(), n_frames, any_pad(ctx, frames, n_frames),
                      total, params, nullptr, nullptr, ctx->iters.as<int>(), ctx->status.as<int>(),
                      ctx->cost.as<float>(), ctx->queue.as<unsigned int>(), s);
   if (rc) {
@@ -646,6 +692,7 @@ extern "C" int ncs_hip_eval_tiles(ncs_ctx *ctx, const float *data, const float *
                                   float alpha) {
   if (!ctx || !data || !gamma || !u || num_tiles < 0) return fail(NCS_E_ARG, "bad arguments");
   if (!ctx->has_mask) return fail(NCS_E_NOMASK, "no OTF mask set");
+  if (ctx->win != 16) return fail(NCS_E_ARG, "tiles are 16 x 16: set a 16 x 16 mask");
   if (num_tiles == 0) return NCS_OK;
   HIPCHK(set_device(ctx));
   hipStream_t s = ctx->stream;
@@ -708,9 +755,10 @@ extern "C" int ncs_hip_net_predict_frames_device(ncs_ctx *ctx, const ncs_frame *
                                                  int n_frames, float *u0_tiles, void *stream) {
   if (!ctx || !frames || n_frames <= 0 || !u0_tiles) return fail(NCS_E_ARG, "bad arguments");
   if (!ctx->has_net) return fail(NCS_E_NOWEIGHTS, "no network weights set");
+  if (ctx->win != 16) return fail(NCS_E_ARG, "the network predicts 16 x 16 tiles");
   std::vector<FrameDev> fd;
   long long total = 0;
-  int rc = build_frames(frames, n_frames, fd, total);
+  int rc = build_frames(ctx, frames, n_frames, fd, total);
   if (rc) return rc;
   HIPCHK(set_device(ctx));
   hipStream_t s = (hipStream_t)stream;
@@ -763,19 +811,21 @@ static ncs_ctx *g_default_ctx = nullptr;
 extern "C" void ncsReduceNoise(double *ncs_image, double *image, double *gamma,
                                double *otf_mask, double alpha, int im_x, int im_y,
                                int r_size) {
-  if (r_size != 16) {
-    fail(NCS_E_ARG, "libncs_hip solves 16 x 16 sub-regions only");
+  // the reference rejects odd sizes (ncs.c:54-58); the GPU solves sides up
+  // to its 16 x 16 tile
+  if (r_size < 4 || r_size > 16 || (r_size & 1)) {
+    fail(NCS_E_ARG, "libncs_hip solves even sub-region sizes 4..16");
     return;
   }
-  if (!ncs_image || !image || !gamma || !otf_mask || im_x < 14 || im_y < 14) {
+  if (!ncs_image || !image || !gamma || !otf_mask || im_x < r_size - 2 || im_y < r_size - 2) {
     fail(NCS_E_ARG, "bad arguments");
     return;
   }
   std::lock_guard<std::mutex> lk(g_default_mu);
   if (!g_default_ctx && ncs_hip_create(0, &g_default_ctx) != NCS_OK) return;
   float m[256];
-  for (int i = 0; i < 256; i++) m[i] = (float)otf_mask[i];
-  if (ncs_hip_set_otf_mask(g_default_ctx, m) != NCS_OK) return;
+  for (int i = 0; i < r_size * r_size; i++) m[i] = (float)otf_mask[i];
+  if (ncs_hip_set_otf_mask_window(g_default_ctx, m, r_size) != NCS_OK) return;
   size_t n = (size_t)im_x * im_y;
   std::vector<float> d(n), g(n), o(n);
   for (size_t i = 0; i < n; i++) {
@@ -808,6 +858,8 @@ extern "C" int ncs_hip_autotune(ncs_ctx *ctx, const ncs_frame *frames, int n_fra
   if (!ctx || !frames || n_frames <= 0 || !res) return fail(NCS_E_ARG, "bad arguments");
   int rc = check_params(base);
   if (rc) return rc;
+  rc = check_window(ctx, base);
+  if (rc) return rc;
   if (!ctx->has_mask) return fail(NCS_E_NOMASK, "no OTF mask set");
   if (base->start != NCS_START_PLAIN && base->start != NCS_START_LINEAR)
     return fail(NCS_E_ARG, "the autotuner times the plain or the linear start");
@@ -815,11 +867,11 @@ extern "C" int ncs_hip_autotune(ncs_ctx *ctx, const ncs_frame *frames, int n_fra
     return fail(NCS_E_NOWEIGHTS, "linear start requested but no coefficients are set");
   std::vector<FrameDev> fd;
   long long total = 0;
-  rc = build_frames(frames, n_frames, fd, total);
+  rc = build_frames(ctx, frames, n_frames, fd, total);
   if (rc) return rc;
   if (total <= 0) return fail(NCS_E_ARG, "empty sample");
   for (FrameDev &f : fd) f.out = nullptr;  // time the solve, write nothing
-  const bool pad = any_pad(frames, n_frames);
+  const bool pad = any_pad(ctx, frames, n_frames);
   HIPCHK(set_device(ctx));
   hipStream_t s = ctx->stream;
   DevBuf dfr, q;
@@ -932,6 +984,7 @@ struct ncs_pipeline {
   size_t pix = 0;
   long long total = 0;
   bool pad = false, general = false, keep = false, learned = false;
+  int win = 16;  // the context's window side when the pipeline was created
   Schedule sched;
   DevBuf raw, data, gamma, out, offset, gain, frames, queue, results, u0;
   // several gain amplifiers: per frame, the number of levels (1 = one gain
@@ -980,6 +1033,8 @@ extern "C" int ncs_hip_pipeline_create(ncs_ctx *ctx, const ncs_pipeline_frame *f
   if (!ctx || !frames || n_frames <= 0) return fail(NCS_E_ARG, "bad arguments");
   int rc = check_params(params);
   if (rc) return rc;
+  rc = check_window(ctx, params);
+  if (rc) return rc;
   if (params->start == NCS_START_TILES)
     return fail(NCS_E_ARG, "a pipeline takes the plain or the learned start");
   if (params->start == NCS_START_LEARNED && !ctx->has_net)
@@ -1003,7 +1058,10 @@ extern "C" int ncs_hip_pipeline_create(ncs_ctx *ctx, const ncs_pipeline_frame *f
   pl->off.resize(n_frames);
   for (int i = 0; i < n_frames; i++) {
     const ncs_pipeline_frame &f = frames[i];
-    long long nt = ncs_hip_tile_count(f.height, f.width, f.geometry);
+    if (ctx->win != 16 && f.geometry != NCS_GEOM_REFERENCE)
+      return fail(NCS_E_ARG, "a " + std::to_string(ctx->win) + " x " + std::to_string(ctx->win) +
+                  " mask takes the reference geometry");
+    long long nt = ncs_hip_tile_count_window(f.height, f.width, f.geometry, ctx->win);
     if (nt < 0) return fail(NCS_E_ARG, "frame " + std::to_string(i) + ": bad size or geometry");
     if (!std::isfinite(f.alpha)) return fail(NCS_E_ARG, "non-finite alpha");
     if ((long long)f.height * f.width >= (1LL << 31))
@@ -1016,11 +1074,11 @@ extern "C" int ncs_hip_pipeline_create(ncs_ctx *ctx, const ncs_pipeline_frame *f
     d.ds = d.gs = d.os = f.width;
     d.alpha = f.alpha;
     d.geom = f.geometry;
-    d.ty = (int)tiles_axis(f.width, f.geometry);
-    d.pad = 0;
+    d.ty = (int)tiles_axis(f.width, f.geometry, ctx->win);
+    d.win = ctx->win;
     d.first = pl->total;
     pl->total += nt;
-    if (f.geometry != NCS_GEOM_REFERENCE) pl->pad = true;
+    if (f.geometry != NCS_GEOM_REFERENCE || ctx->win != 16) pl->pad = true;
   }
   if (pl->total > 0xF0000000LL) return fail(NCS_E_ARG, "too many tiles in one batch");
   HIPCHK(set_device(ctx));
@@ -1054,6 +1112,7 @@ extern "C" int ncs_hip_pipeline_create(ncs_ctx *ctx, const ncs_pipeline_frame *f
   pl->levels.assign(n_frames, 1);
   pl->level_points.resize(n_frames);
   pl->sched = resolve_schedule(ctx, params, pl->pad);
+  pl->win = ctx->win;
   *out = pl.release();
   return NCS_OK;
 }
@@ -1270,6 +1329,9 @@ extern "C" int ncs_hip_pipeline_launch(ncs_pipeline *pl, const void *const *inpu
   if (pl->exec && ctx->general != pl->general)
     return fail(NCS_E_MASK, "the kind of the OTF mask changed since the pipeline was "
                 "captured; create a new pipeline");
+  if (ctx->win != pl->win)
+    return fail(NCS_E_MASK, "the mask size changed since the pipeline was created; create a "
+                "new pipeline");
   int rc;
   if (!pl->exec) {
     rc = build_pipeline_graph(pl);

@@ -47,7 +47,8 @@ struct FrameDev {
   float alpha;
   int geom;            // NCS_GEOM_*
   int ty;              // tiles along the fast axis
-  int pad;
+  int win;             // window side r of the reference geometry (even, 4..16);
+                       // a window fills the top-left r x r of the 16 x 16 tile
   long long first;     // global index of the frame's first tile
 };
 

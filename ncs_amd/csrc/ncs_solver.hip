@@ -573,13 +573,17 @@ __device__ __forceinline__ TileGeom tile_geom(const FrameConst &F, long long loc
   const unsigned int l32 = (unsigned int)local, ty = (unsigned int)F.ty;
   const int a = (int)(l32 / ty), b = (int)(l32 - (unsigned int)a * ty);
   if (F.geom == NCS_GEOM_REFERENCE) {
-    const int lastx = F.H - 15, lasty = F.W - 15;
-    G.bx = min(-1 + 14 * a, lastx);
-    G.by = min(-1 + 14 * b, lasty);
+    // windows of side r at -1, -1 + (r - 2), ...; the last one clamped to
+    // end one pixel past the frame (ncs.c:71-85); each owns its interior up
+    // to the next window's interior
+    const int r = F.win, s = r - 2;
+    const int lastx = F.H + 1 - r, lasty = F.W + 1 - r;
+    G.bx = min(-1 + s * a, lastx);
+    G.by = min(-1 + s * b, lasty);
     G.r_lo = 1;
-    G.r_hi = (G.bx == lastx) ? 15 : min(15, lastx + 1 - G.bx);
+    G.r_hi = (G.bx == lastx) ? r - 1 : min(r - 1, lastx + 1 - G.bx);
     G.c_lo = 1;
-    G.c_hi = (G.by == lasty) ? 15 : min(15, lasty + 1 - G.by);
+    G.c_hi = (G.by == lasty) ? r - 1 : min(r - 1, lasty + 1 - G.by);
   } else {
     G.bx = 16 * a;
     G.by = 16 * b;
@@ -775,11 +779,18 @@ void solve_kernel(SolveArgs A) {
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       const int row = 4 * grp + r;
-      if (!PAD || F.geom == NCS_GEOM_REFERENCE) {
+      if (!PAD) {
         const unsigned int gr = (unsigned int)min(max(G.bx + row, 0), F.H - 1);
         const unsigned int gc = (unsigned int)min(max(G.by + col, 0), F.W - 1);
         dd[r] = F.data[gr * (unsigned int)F.ds + gc];
         gg[r] = F.gamma[gr * (unsigned int)F.gs + gc];
+      } else if (F.geom == NCS_GEOM_REFERENCE) {
+        // an r x r window in the top-left of the tile, zeros around it
+        const bool in = (row < F.win) && (col < F.win);
+        const unsigned int gr = (unsigned int)min(max(G.bx + row, 0), F.H - 1);
+        const unsigned int gc = (unsigned int)min(max(G.by + col, 0), F.W - 1);
+        dd[r] = in ? F.data[gr * (unsigned int)F.ds + gc] : 0.0f;
+        gg[r] = in ? F.gamma[gr * (unsigned int)F.gs + gc] : 0.0f;
       } else {
         const bool ok = (row < G.r_hi) && (col < G.c_hi);
         const unsigned int gr = ok ? (unsigned int)(G.bx + row) : 0u;
@@ -809,7 +820,8 @@ void solve_kernel(SolveArgs A) {
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       const int row = 4 * grp + r;
-      lin[r] = (!PAD || F.geom == NCS_GEOM_REFERENCE || (row < G.r_hi && col < G.c_hi))
+      lin[r] = (!PAD || (F.geom == NCS_GEOM_REFERENCE ? (row < F.win && col < F.win)
+                                                      : (row < G.r_hi && col < G.c_hi)))
                    ? 1.0f : 0.0f;
       c[r] = lin[r] * (raw_d[r] + gm[r]);
       u[r] = raw_d[r];

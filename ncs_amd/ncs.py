@@ -58,9 +58,11 @@ def _geometry_code(geometry):
 class NCSHip(object):
     """The class of opencl/pyOpenCLNCS/ncs.py:39-222 on libncs_hip."""
 
-    def __init__(self, strict=True, device=0, params=None, **kwds):
+    def __init__(self, strict=True, device=0, params=None, size=16, **kwds):
         super().__init__(**kwds)
-        self.size = 16
+        if size < 4 or size > 16 or size % 2:
+            raise NCSHipException("the GPU solver takes even sub-region sizes 4..16")
+        self.size = size
         self.strict = strict
         self.ctx = _lib.context(device)
         self.params = params if params is not None else _lib.default_params()
@@ -172,11 +174,13 @@ def cReduceNoise(image, gamma, otf_mask, alpha, strict=True, *, params=None, dev
             raise NCSHipException("OTF must be square!")
         if (otf_mask.shape[0] % 2) != 0:
             raise NCSHipException("Sub region size must be divisible by 2!")
-    if otf_mask.shape != (16, 16):
-        raise NCSHipException("the GPU solver takes 16 x 16 masks")
-    centred = numpy.fft.ifftshift(otf_mask)
-    out = reduceNoise([numpy.asarray(image)], numpy.asarray(gamma), centred, alpha,
-                      strict=False, params=params, device=device)
+    r = otf_mask.shape[0]
+    if otf_mask.shape != (r, r) or r < 4 or r > 16 or r % 2:
+        raise NCSHipException("the GPU solver takes even sub-region sizes 4..16")
+    ncs = NCSHip(strict=False, device=device, params=params, size=r)
+    ncs.setGamma(numpy.asarray(gamma))
+    ncs.setOTFMask(numpy.fft.ifftshift(otf_mask))
+    out = ncs.reduceNoise([numpy.asarray(image)], alpha)
     return out[0].astype(numpy.float64)
 
 

@@ -70,9 +70,10 @@ def test_adu_to_photons_clamps_and_gamma():
 
 
 def _reference_windows(img, r=16):
-    """Windows of the reference tiling: starts -1, 13, ... while the window
-    ends inside the padded image, the last one clamped to H + 1 - r, pixels
-    outside the image replaced by the edge (ncs.c:71-113)."""
+    """Windows of the reference tiling: starts -1, 13, ..., the last one
+    clamped to H + 1 - r and the loop ending after the window at that
+    position; pixels outside the image replaced by the edge
+    (ncs.c:71-113, 136-142)."""
     H = img.shape[0]
     starts = []
     i = -1
@@ -91,8 +92,8 @@ def test_segpadimg_r14_windows_are_the_reference_windows():
     segs = camera.segpadimg(img, 14)
     assert segs.shape == (64, 16, 16)
     starts, ref = _reference_windows(img)
-    # eight block windows per axis; the reference's last (clamped) window
-    # repeats the eighth
+    # eight windows per axis in both: the eighth ends exactly one pixel past
+    # the edge, and the reference stops after the window at that position
     assert starts == [-1 + 14 * k for k in range(8)]
     for bj in range(8):
         for bi in range(8):

@@ -203,3 +203,73 @@ def test_pyncs_r14_segmentation_is_the_reference_geometry():
     seg = camera.stitchpadimg(u)
     _say("max |diff|", float(numpy.max(numpy.abs(seg - out))), "status", numpy.unique(st))
     assert numpy.array_equal(seg.astype(numpy.float32), out)
+
+
+@pytest.fixture
+def _restore_16(gpu_ctx):
+    yield gpu_ctx
+    gpu_ctx.set_otf_mask(masks.to_fft_order(masks.otf_mask()).astype(numpy.float32))
+
+
+@pytest.mark.parametrize("r", [10, 12])
+def test_window_side_r_alpha_zero_is_bit_exact_identity(r, _restore_16):
+    """Sub-regions of side r (clib ncs.c:41-147 with r_size = r): at alpha = 0
+    the image comes back bit for bit, which pins the r-windows' gather,
+    clamp and stitch, down to frames of side r - 2."""
+    _say("window", r, "identity")
+    rng = numpy.random.default_rng(r)
+    for shape in [(r - 2, r - 2), (r - 1, 37), (50, 61)]:
+        image = rng.uniform(0.5, 100, shape).astype(numpy.float32)
+        gamma = rng.uniform(1, 10, shape).astype(numpy.float32)
+        mask = masks.to_fft_order(masks.otf_mask(r=r))
+        out = ncs.cReduceNoise(image, gamma, mask, 0.0)
+        assert numpy.array_equal(out.astype(numpy.float32), image), shape
+
+
+@pytest.mark.parametrize("r", [10, 12])
+def test_window_side_r_parseval_closed_form(r, _restore_16):
+    """All-ones r x r mask: NC = sum u^2 in the r-point basis, so every pixel
+    has the closed-form minimiser whatever the tiling."""
+    _say("window", r, "parseval")
+    rng = numpy.random.default_rng(40 + r)
+    d = rng.uniform(5, 60, (45, 53)).astype(numpy.float32)
+    g = rng.uniform(1, 20, (45, 53)).astype(numpy.float32)
+    a = 0.1
+    c = d.astype(numpy.float64) + g
+    expect = (-(1 + 2 * a * g) + numpy.sqrt((1 + 2 * a * g) ** 2 - 8 * a * (g - c))) / (4 * a)
+    h = ncs.NCSHip(size=r, params=_lib.default_params(epsilon=1e-7))
+    h.setOTFMask(numpy.ones((r, r)))
+    h.setGamma(g)
+    out, info = h.reduceNoise([d], a, return_info=True)
+    err = float(numpy.max(numpy.abs(out[0] - expect)))
+    _say("max abs err", err, "status", numpy.unique(info["status"]))
+    assert err < 2.5e-4
+
+
+@pytest.mark.parametrize("r", [10, 12])
+def test_window_side_r_tile_costs_match_oracle(r, _restore_16):
+    """A 96 x 96 synthetic frame with an r x r OTF mask: per-tile final costs
+    (FP64 oracle with the same r) within 1e-5, tile counts equal."""
+    _say("window", r, "oracle")
+    fr = sim.make_frame(90 + r, (96, 96))
+    mask = masks.otf_mask(r=r)
+    h = ncs.NCSHip(size=r)
+    h.setOTFMask(mask)
+    h.setGamma(fr["gamma"])
+    out, info = h.reduceNoise([fr["data"]], 0.1, return_info=True)
+    ref, rinfo = oracle.reduce_noise(fr["data"].astype(numpy.float64),
+                                     fr["gamma"].astype(numpy.float64),
+                                     masks.to_fft_order(mask), 0.1)
+    assert len(info["cost"]) == len(rinfo["cost"]) == _lib.tile_count(96, 96, window=r)
+    rel = numpy.abs(info["cost"].astype(numpy.float64) - rinfo["cost"]) / numpy.abs(rinfo["cost"])
+    _say("tiles", len(rel), "cost max rel", float(rel.max()), "pixel max abs diff",
+         float(numpy.max(numpy.abs(out[0] - ref))))
+    assert numpy.all(info["status"] == 0)
+    assert rel.max() <= 1e-5
+    with pytest.raises(_lib.NCSHipError):  # 16 x 16 tiles only
+        _lib.context(0).solve_tiles(numpy.ones((2, 16, 16), numpy.float32),
+                                    numpy.ones((2, 16, 16), numpy.float32), 0.1,
+                                    _lib.default_params())
+    with pytest.raises(_lib.NCSHipError):  # the plain start only
+        h.params = _lib.default_params(start=_lib.START_LINEAR)
+        h.reduceNoise([fr["data"]], 0.1)

@@ -76,11 +76,34 @@ def test_context_on_a_machine_without_gpu_reports_an_error():
     assert "device" in _lib.last_error().lower()
 
 
-def test_drop_in_rejects_other_subregion_sizes_without_touching_memory():
+def test_drop_in_rejects_unsupported_subregions_without_touching_memory():
+    """Odd sizes (the reference refuses them, ncs.c:54-58), sizes past the
+    16 x 16 tile and frames smaller than r - 2 (where the reference would
+    write out of bounds) are refused before any device work."""
     lib = _lib.load()
-    out = numpy.full((20, 20), 7.0)
     img = numpy.ones((20, 20))
-    lib.ncsReduceNoise(out.ctypes.data, img.ctypes.data, img.ctypes.data,
-                       numpy.ones((10, 10)).ctypes.data, 0.1, 20, 20, 10)
-    assert numpy.all(out == 7.0)
-    assert "16" in _lib.last_error()
+    for r, h in ((9, 20), (18, 20), (12, 9)):
+        out = numpy.full((20, 20), 7.0)
+        lib.ncsReduceNoise(out.ctypes.data, img.ctypes.data, img.ctypes.data,
+                           numpy.ones((r, r)).ctypes.data, 0.1, h, 20, r)
+        assert numpy.all(out == 7.0), r
+        err = _lib.last_error()
+        assert ("16" in err) if r != 12 else ("bad arguments" in err), (r, err)
+
+
+def test_tile_counts_follow_the_window_side():
+    """ncs.c:71-143 with windows of side r: starts -1, -1 + (r - 2), ...
+    until a window ends at or past the padded edge (that one clamped)."""
+    def loop(n, r):
+        count, i = 0, -1
+        while True:
+            count += 1
+            if i + r >= n + 1:
+                return count
+            i += r - 2
+    for r in (4, 6, 10, 12, 16):
+        for n in (r - 2, r - 1, r, 30, 31, 97, 128):
+            assert _lib.tile_count(n, n, _lib.GEOM_REFERENCE, r) == loop(n, r) ** 2, (r, n)
+        assert _lib.tile_count(r - 3, 40, _lib.GEOM_REFERENCE, r) == -1
+    assert _lib.tile_count(40, 40, _lib.GEOM_BLOCKS, 12) == -1
+    assert _lib.tile_count(40, 40, _lib.GEOM_REFERENCE, 7) == -1
