@@ -191,6 +191,48 @@ def test_pipeline_gain_levels_pick_maps_by_adu(otf):
     pl.close()
 
 
+def test_pipeline_gain_levels_per_frame(otf):
+    """Two frames of different sizes in one pipeline, the first with two
+    gain levels and the second with one map: each frame's photons and gamma
+    come from its own maps (the level maps sit at each frame's offset in the
+    packed buffers)."""
+    from oracle import camera
+    ctx = otf
+    pin = _Pinned()
+    shapes = [(40, 50), (33, 64)]
+    fr = [sim.make_frame(45 + i, s) for i, s in enumerate(shapes)]
+    p = _lib.default_params()
+    pl = _lib.Pipeline(ctx, shapes, [0.1, 0.2], params=p, input_format=_lib.INPUT_ADU_U16,
+                       keep_tile_results=True)
+    srcs, outs, phs, gms = [], [], [], []
+    for i, f in enumerate(fr):
+        adu = numpy.clip(numpy.rint(f["adu"]), 0, 65535).astype(numpy.uint16)
+        off = f["offset"].astype(numpy.float32)
+        if i == 0:
+            gains = numpy.stack([f["gain"], 1.7 * f["gain"]]).astype(numpy.float32)
+            var = numpy.stack([f["var"], 3.0 * f["var"]]).astype(numpy.float32)
+            pt = [float(numpy.median(adu))]
+            pl.set_camera_levels(0, pt, off, gains, var)
+            gs, vs = camera.genmaps(adu[None], gains, var, pt)
+            g0, v0 = gs[0], vs[0]
+        else:
+            g0, v0 = f["gain"].astype(numpy.float32), f["var"].astype(numpy.float32)
+            pl.set_camera(1, off, g0, v0)
+        phs.append(camera.adu_to_photons(adu, off, g0))
+        gms.append(camera.gamma_from_camera(v0, g0))
+        srcs.append(pin(shapes[i], numpy.uint16))
+        srcs[-1][...] = adu
+        outs.append(pin(shapes[i], numpy.float32))
+    ref = _eager(ctx, phs, gms, [0.1, 0.2], p)
+    for k in range(2):
+        pl.launch(srcs, outs)
+        pl.wait()
+        for i in range(2):
+            assert numpy.array_equal(outs[i], ref[0][i]), (k, i)
+        assert numpy.array_equal(pl.tile_results()[2], ref[3])
+    pl.close()
+
+
 def test_pipeline_blocks_geometry_uneven_frames(otf):
     ctx = otf
     pin = _Pinned()
