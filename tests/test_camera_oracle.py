@@ -100,6 +100,21 @@ def test_segpadimg_r14_windows_are_the_reference_windows():
             assert numpy.array_equal(segs[bj * 8 + bi], ref[(starts[bi], starts[bj])]), (bi, bj)
 
 
+def test_segpadimg_blocks_are_reference_windows_of_side_r1_plus_2():
+    """pyNCS's blocks of R1 = 8 and 10 (windows of 10 and 12) are the
+    reference tiling's windows of side R1 + 2 whenever R1 divides the side:
+    the frame path with that window side serves them."""
+    for r1, R in ((8, 48), (10, 60), (10, 30)):
+        img = numpy.random.default_rng(r1 + R).uniform(0, 100, (R, R))
+        segs = camera.segpadimg(img, r1)
+        ns = R // r1
+        starts, ref = _reference_windows(img, r1 + 2)
+        assert starts == [-1 + r1 * k for k in range(ns)], (r1, R)
+        for bj in range(ns):
+            for bi in range(ns):
+                assert numpy.array_equal(segs[bj * ns + bi], ref[(starts[bi], starts[bj])])
+
+
 def test_stitchpadimg_places_interiors():
     img = numpy.random.default_rng(4).uniform(0, 100, (60, 60))
     assert numpy.array_equal(camera.stitchpadimg(camera.segpadimg(img, 10)), img)
