@@ -1033,6 +1033,38 @@ void solve_kernel(SolveArgs A) {
           }
         }
       }
+      if (ls != LS_ACCEPT && !exact) {
+        // The FP16 split's error can stall the search on an ill-conditioned
+        // tile, where the likelihood and NC parts of phi'(t) nearly cancel:
+        // the tile goes to FP32 basis changes, drops its history, refreshes
+        // its spectral gradient exactly and searches again from u.
+        exact = true;
+        nh = 0;
+        to_spec16<false>(u, T, GB, st, zs, exact);
+        nc_spec<GEN>(zs, T, gncs);
+        float gp[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+          gp[r] = (PAD ? lin[r] : 1.0f) -
+                  c[r] * (kCacheIv ? iv[r] : rcpf(fmaxf(u[r] + gm[r], kFitMin)));
+        to_spec16(gp, T, GB, 1.0f, gs, exact);
+        float red[1] = {0.0f};
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          gs[r] = fmaf(alpha, gncs[r], gs[r]);
+          ds[r] = -gs[r];
+          red[0] = fmaf(gs[r], gs[r], red[0]);
+        }
+        wave_sum<1>(red);
+        gg = red[0];
+        if (spec_h0) {
+          h0_spec(ds, T, kappa, alpha);
+          step = 1.0f;
+        } else {
+          step = rsqrtf(gg);
+        }
+        continue;
+      }
       if (ls != LS_ACCEPT && spec_h0) {  // retry with the scalar H0
         spec_h0 = false;
         nh = 0;

@@ -246,25 +246,41 @@ def test_window_side_r_parseval_closed_form(r, _restore_16):
     assert err < 2.5e-4
 
 
+@pytest.mark.parametrize("kind", ["otf", "random"])
 @pytest.mark.parametrize("r", [10, 12])
-def test_window_side_r_tile_costs_match_oracle(r, _restore_16):
-    """A 96 x 96 synthetic frame with an r x r OTF mask: per-tile final costs
-    (FP64 oracle with the same r) within 1e-5, tile counts equal."""
-    _say("window", r, "oracle")
-    fr = sim.make_frame(90 + r, (96, 96))
-    mask = masks.otf_mask(r=r)
+def test_window_side_r_tile_costs_match_oracle(r, kind, _restore_16):
+    """96 x 96 frames with an r x r mask: per-tile final costs (FP64 oracle
+    with the same r) within 1e-5, tile counts equal.  The OTF mask runs on a
+    synthetic filament scene at alpha 0.1.  The random point-symmetric mask
+    has an odd-odd part, so it exercises the partner coupling of the r-point
+    basis (the general-mask build); it runs on the inputs the reference's own
+    random-mask test uses (opencl/pyOpenCLNCS/test/test_im.py:16-44: image
+    U(0.01, 10), gamma U(2, 4), alpha 0.02).  On bright scenes such masks let
+    pixels reach the FITMIN guard, where the reference gradient is not the
+    cost's derivative, and the FP32 and FP64 solvers may stop at different
+    points."""
+    _say("window", r, kind, "oracle")
+    if kind == "otf":
+        fr = sim.make_frame(90 + r, (96, 96))
+        data, gamma, mask, alpha = fr["data"], fr["gamma"], masks.otf_mask(r=r), 0.1
+    else:
+        rng = numpy.random.default_rng(70 + r)
+        data = rng.uniform(0.01, 10.0, (96, 96)).astype(numpy.float32)
+        gamma = rng.uniform(2.0, 4.0, (96, 96)).astype(numpy.float32)
+        mask, alpha = masks.random_mask(r, numpy.random.default_rng(r)), 0.02
     h = ncs.NCSHip(size=r)
     h.setOTFMask(mask)
-    h.setGamma(fr["gamma"])
-    out, info = h.reduceNoise([fr["data"]], 0.1, return_info=True)
-    ref, rinfo = oracle.reduce_noise(fr["data"].astype(numpy.float64),
-                                     fr["gamma"].astype(numpy.float64),
-                                     masks.to_fft_order(mask), 0.1)
+    h.setGamma(gamma)
+    out, info = h.reduceNoise([data], alpha, return_info=True)
+    ref, rinfo = oracle.reduce_noise(data.astype(numpy.float64), gamma.astype(numpy.float64),
+                                     masks.to_fft_order(mask), alpha)
     assert len(info["cost"]) == len(rinfo["cost"]) == _lib.tile_count(96, 96, window=r)
     rel = numpy.abs(info["cost"].astype(numpy.float64) - rinfo["cost"]) / numpy.abs(rinfo["cost"])
     _say("tiles", len(rel), "cost max rel", float(rel.max()), "pixel max abs diff",
          float(numpy.max(numpy.abs(out[0] - ref))))
-    assert numpy.all(info["status"] == 0)
+    # -5 (no progress at FP32 precision) is not a failure for the reference
+    # either (pyOpenCLNCS/ncs.py:159-162); the cost bar decides
+    assert numpy.all((info["status"] == 0) | (info["status"] == -5))
     assert rel.max() <= 1e-5
     with pytest.raises(_lib.NCSHipError):  # 16 x 16 tiles only
         _lib.context(0).solve_tiles(numpy.ones((2, 16, 16), numpy.float32),
@@ -272,4 +288,4 @@ def test_window_side_r_tile_costs_match_oracle(r, _restore_16):
                                     _lib.default_params())
     with pytest.raises(_lib.NCSHipError):  # the plain start only
         h.params = _lib.default_params(start=_lib.START_LINEAR)
-        h.reduceNoise([fr["data"]], 0.1)
+        h.reduceNoise([data], alpha)

@@ -120,3 +120,30 @@ def test_tight_tolerance_finishes_in_fp32(gpu_ctx):
          float(numpy.max(gn / numpy.maximum(1.0, un))))
     assert numpy.all(st == 0)
     assert numpy.all(gn <= 5e-7 * numpy.maximum(1.0, un))
+
+
+def test_stalled_fp16_search_retries_in_fp32(gpu_ctx):
+    """A bright scene under a random 10 x 10 mask holds a tile whose line
+    search, in FP16 basis changes, stalls far from the minimum (the
+    likelihood and NC parts of phi'(t) nearly cancel).  The tile must retry
+    in FP32 and reach the FP64 oracle's cost like every other tile."""
+    from ncs_amd import ncs
+    r = 10
+    fr = sim.make_frame(100, (96, 96))
+    mask = masks.random_mask(r, numpy.random.default_rng(r))
+    try:
+        h = ncs.NCSHip(size=r)
+        h.setOTFMask(mask)
+        h.setGamma(fr["gamma"])
+        out, info = h.reduceNoise([fr["data"]], 0.1, return_info=True)
+    finally:
+        gpu_ctx.set_otf_mask(masks.to_fft_order(masks.otf_mask()).astype(numpy.float32))
+    ref, rinfo = oracle.reduce_noise(fr["data"].astype(numpy.float64),
+                                     fr["gamma"].astype(numpy.float64),
+                                     masks.to_fft_order(mask), 0.1)
+    rel = numpy.abs(info["cost"].astype(numpy.float64) - rinfo["cost"]) / numpy.abs(rinfo["cost"])
+    _say("random 10x10 mask, bright scene: status",
+         dict(zip(*[x.tolist() for x in numpy.unique(info["status"], return_counts=True)])),
+         "max rel cost", float(rel.max()))
+    assert numpy.all(info["status"] == 0)
+    assert rel.max() <= 1e-5
