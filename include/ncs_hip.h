@@ -42,13 +42,24 @@ extern "C" {
 #define NCS_GEOM_REFERENCE 0 /* 16x16 windows, stride 14, 1-px replicated halo,
                                 last window clamped, interiors stitched with the
                                 later window owning overlaps (ncs.c:71-143) */
-#define NCS_GEOM_BLOCKS 1    /* non-overlapping 16x16 blocks; pixels outside the
-                                frame are zero-padded and excluded from the
-                                likelihood term */
+#define NCS_GEOM_BLOCKS 1    /* non-overlapping blocks of the window side (16,
+                                or r after ncs_hip_set_otf_mask_window); pixels
+                                outside the frame are zero-padded and excluded
+                                from the likelihood term */
 
 /* line searches */
 #define NCS_LS_MORE_THUENTE 0 /* liblbfgs default (lbfgs.h:704-719) */
 #define NCS_LS_ARMIJO 1       /* backtracking of opencl/ncs.cl:945-1035 */
+
+/* arithmetic of the solver's basis changes (the 16x16 real-DFT transforms) */
+#define NCS_PRECISION_FP32 0      /* FP32 MFMA (v_mfma_f32_16x16x4_f32) throughout:
+                                     every operand and accumulator FP32 (default) */
+#define NCS_PRECISION_F16_SPLIT 1 /* each FP32 operand split into two FP16 halves
+                                     on the FP16 matrix cores, FP32 accumulation
+                                     (about 2^-22 per operand); FP32 MFMA once the
+                                     gradient is within 1e-5 of the stop rule's
+                                     scale, on overflow, and for the rest of a
+                                     tile whose line search fails */
 
 /* starting point */
 #define NCS_START_PLAIN 0   /* u0 = data (ncs.c:549-551, ncs.cl:906) */
@@ -80,6 +91,7 @@ typedef struct {
                           scalar s.y/y.y of liblbfgs and ncs.cl:1123 */
   int waves_per_group; /* tiles (waves) per workgroup, 1..4; 0 = autotuned
                           (else 1) */
+  int precision;       /* NCS_PRECISION_* */
 } ncs_params;
 
 typedef struct {
@@ -99,7 +111,7 @@ typedef struct ncs_ctx ncs_ctx;
 /* Fill defaults: max_iterations 200, history 5, epsilon 1e-5, More-Thuente
  * (ftol 1e-4, gtol 0.9, xtol 1e-7, max_linesearch 20, steps 1e-20..1e20),
  * Armijo stepm 0.5, plain start, fallback on, autotuned grid and workgroup
- * size, spectral H0. */
+ * size, spectral H0, FP32 basis changes. */
 void ncs_hip_default_params(ncs_params *p);
 
 int ncs_hip_create(int device, ncs_ctx **out);
@@ -117,19 +129,20 @@ int ncs_hip_set_otf_mask(ncs_ctx *ctx, const float *mask_fft_order);
 
 /* An r x r mask (FFT order, r even, 4..16) for sub-regions of side r, the
  * clib's r_size (ncs.c:41-147 takes any even r; pyCNCS derives it from the
- * mask, ncs_c.py:250-276).  Frames then use the reference geometry with
- * windows of side r, stride r - 2, each solved in the top-left of a 16 x 16
- * tile whose other pixels are left out of the cost; only the plain start is
- * accepted, and the tile and network entry points, which take 16 x 16
- * tiles, are refused.  r = 16 is ncs_hip_set_otf_mask. */
+ * mask, ncs_c.py:250-276).  Frames then use windows (reference geometry:
+ * stride r - 2) or blocks of side r, each solved in the top-left of a
+ * 16 x 16 tile whose other pixels are left out of the cost, and the tile
+ * entry points take r x r tiles; only the plain start is accepted, and the
+ * evaluation and network entry points, which take 16 x 16 tiles, are
+ * refused.  r = 16 is ncs_hip_set_otf_mask. */
 int ncs_hip_set_otf_mask_window(ncs_ctx *ctx, const float *mask_fft_order, int r);
 
 /* Number of tiles a frame produces with the given geometry (16 x 16
  * windows); -1 if the frame is too small.  Per-tile result arrays hold
  * this many entries. */
 long long ncs_hip_tile_count(int height, int width, int geometry);
-/* The same for windows of side r (the reference geometry only for r < 16):
- * the count after ncs_hip_set_otf_mask_window(ctx, mask, r). */
+/* The same for windows or blocks of side r: the count after
+ * ncs_hip_set_otf_mask_window(ctx, mask, r). */
 long long ncs_hip_tile_count_window(int height, int width, int geometry, int r);
 
 /* Whole frames, host memory in and out; synchronous.  Replaces the batched
@@ -150,10 +163,11 @@ int ncs_hip_reduce_frames_device(ncs_ctx *ctx, const ncs_frame *frames,
                                  const float *u0_tiles, int *iterations,
                                  int *status, float *cost, void *stream);
 
-/* Independent 16x16 tiles [num_tiles][16][16], host memory; synchronous.
- * Replaces the kernel ABI ncsReduceNoise(data_in, g_gamma, otf_mask,
- * data_out, g_iterations, g_status, alpha) of opencl/ncs.cl:820-826 (the
- * mask comes from ncs_hip_set_otf_mask).  cost may be NULL. */
+/* Independent 16x16 tiles [num_tiles][16][16] (r x r tiles [num_tiles][r][r]
+ * after an r x r mask), host memory; synchronous.  Replaces the kernel ABI
+ * ncsReduceNoise(data_in, g_gamma, otf_mask, data_out, g_iterations,
+ * g_status, alpha) of opencl/ncs.cl:820-826 (the mask comes from
+ * ncs_hip_set_otf_mask).  cost may be NULL. */
 int ncs_hip_solve_tiles(ncs_ctx *ctx, const float *data_in,
                         const float *gamma, float *data_out, int *iterations,
                         int *status, float *cost, int num_tiles, float alpha,
@@ -182,6 +196,48 @@ int ncs_hip_eval_tiles(ncs_ctx *ctx, const float *data, const float *gamma,
 void ncsReduceNoise(double *ncs_image, double *image, double *gamma,
                     double *otf_mask, double alpha, int im_x, int im_y,
                     int r_size);
+
+/* ---- per-sub-region C ABI of the C library (clib/pyCNCS/ncs.h:43-53) ----
+ *
+ * Same names, signatures and call conventions as the reference, so that a
+ * pyCNCS-style ctypes binding (clib/pyCNCS/ncs_c.py:18-60) loads this
+ * library unchanged.  A sub-region keeps its r x r data, gamma, iterate and
+ * mask in host memory (the caller's arrays are copied, float64 row-major)
+ * and owns a solver context on GPU 0; one sub-region per host thread, as in
+ * the reference.  The sizes are even, 4..16 (the reference takes any even
+ * size).  The ncsSRCalc* functions evaluate in FP64 on the GPU; ncsSRSolve
+ * runs the FP32 solver with liblbfgs's semantics. */
+typedef struct ncsSubRegion ncsSubRegion;
+
+/* ncs.c:394-446.  NULL for an unsupported size or without a GPU (message in
+ * ncs_hip_last_error()). */
+ncsSubRegion *ncsSRInitialize(int r_size);
+/* ncs.c:459-468: copy the sub-region's data and gamma (r x r each). */
+void ncsSRNewRegion(ncsSubRegion *sr, double *image, double *gamma);
+/* ncs.c:502-510: r x r mask, FFT order, unsquared. */
+void ncsSRSetOTFMask(ncsSubRegion *sr, double *otf_mask);
+/* ncs.c:521-529 and 375-383: set / read the iterate u. */
+void ncsSRSetU(ncsSubRegion *sr, double *u);
+void ncsSRGetU(ncsSubRegion *sr, double *u);
+/* ncs.c:539-561: u = data, then L-BFGS (history 6, epsilon 1e-5,
+ * More-Thuente, initial Hessian s.y/y.y, no iteration cap; FP32 on the GPU).
+ * Returns liblbfgs's code (lbfgs.h:72-147): 0 converged, 2 already minimal,
+ * -1000 minimum step, -994 not a descent direction, -1001 no representable
+ * progress, -1024 a library error.  verbose prints the final iteration's
+ * norms (the reference prints every iteration, ncs.c:476-491). */
+int ncsSRSolve(ncsSubRegion *sr, double alpha, int verbose);
+/* ncs.c:185-208: sum of u - (data + gamma) ln max(u + gamma, 1e-6). */
+double ncsSRCalcLogLikelihood(ncsSubRegion *sr);
+/* ncs.c:158-175: 1 - (data + gamma) / max(u + gamma, 1e-6), r x r. */
+void ncsSRCalcLLGradient(ncsSubRegion *sr, double *gradient);
+/* ncs.c:259-299: (1/r^2) sum_k mask_k^2 |DFT2(u)_k|^2. */
+double ncsSRCalcNoiseContribution(ncsSubRegion *sr);
+/* ncs.c:220-250: c2r of 2 mask^2 DFT2(u) on the half spectrum, / r^2.  It
+ * transforms the current u (the reference reuses the transform of the last
+ * ncsSRCalcNoiseContribution: the same in its call order). */
+void ncsSRCalcNCGradient(ncsSubRegion *sr, double *gradient);
+/* ncs.c:307-327. */
+void ncsSRCleanup(ncsSubRegion *sr);
 
 /* ---- learned warm start: 4-layer 3x3 conv net on BF16 MFMA ---- */
 

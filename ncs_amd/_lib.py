@@ -29,6 +29,8 @@ START_PLAIN = 0
 START_TILES = 1
 START_LEARNED = 2
 START_LINEAR = 3
+PRECISION_FP32 = 0
+PRECISION_F16_SPLIT = 1
 NET_PARAMS = 19393
 LINEAR_START_PARAMS = 513
 INPUT_PHOTONS_F32 = 0
@@ -58,7 +60,8 @@ class Params(ctypes.Structure):
                 ("fallback", ctypes.c_int),
                 ("waves_per_cu", ctypes.c_int),
                 ("precond", ctypes.c_int),
-                ("waves_per_group", ctypes.c_int)]
+                ("waves_per_group", ctypes.c_int),
+                ("precision", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -127,6 +130,17 @@ SIGNATURES = {
     "ncs_hip_net_forward": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _vp]),
     "ncs_hip_net_backward": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp]),
     "ncsReduceNoise": (None, [_vp, _vp, _vp, _vp, ctypes.c_double, _i, _i, _i]),
+    "ncsSRInitialize": (_vp, [_i]),
+    "ncsSRNewRegion": (None, [_vp, _vp, _vp]),
+    "ncsSRSetOTFMask": (None, [_vp, _vp]),
+    "ncsSRSetU": (None, [_vp, _vp]),
+    "ncsSRGetU": (None, [_vp, _vp]),
+    "ncsSRSolve": (_i, [_vp, ctypes.c_double, _i]),
+    "ncsSRCalcLogLikelihood": (ctypes.c_double, [_vp]),
+    "ncsSRCalcLLGradient": (None, [_vp, _vp]),
+    "ncsSRCalcNoiseContribution": (ctypes.c_double, [_vp]),
+    "ncsSRCalcNCGradient": (None, [_vp, _vp]),
+    "ncsSRCleanup": (None, [_vp]),
     "ncs_hip_autotune": (_i, [_vp, _F, _i, _P, _i, ctypes.POINTER(TuneResult)]),
     "ncs_hip_pipeline_create": (_i, [_vp, ctypes.POINTER(PipelineFrame), _i, _P, _i, _i,
                                      ctypes.POINTER(_vp)]),
@@ -307,11 +321,13 @@ class Context(object):
         return iters, status, cost
 
     def solve_tiles(self, data, gamma, alpha, params):
-        """Independent tiles [n, 16, 16] (host); returns (u, iters, status, cost)."""
+        """Independent tiles [n, r, r] (host; r the mask's side, 16 unless an
+        r x r mask is set); returns (u, iters, status, cost)."""
         d = numpy.ascontiguousarray(data, dtype=numpy.float32)
         g = numpy.ascontiguousarray(gamma, dtype=numpy.float32)
-        if d.ndim != 3 or d.shape[1:] != (16, 16) or g.shape != d.shape:
-            raise ValueError("tiles must be [n, 16, 16] with matching gamma")
+        r = getattr(self, "window", 16)
+        if d.ndim != 3 or d.shape[1:] != (r, r) or g.shape != d.shape:
+            raise ValueError("tiles must be [n, %d, %d] with matching gamma" % (r, r))
         n = d.shape[0]
         u = numpy.zeros_like(d)
         iters = numpy.zeros(n, numpy.int32)

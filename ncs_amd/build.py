@@ -22,7 +22,8 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-ffp-contract=fast"]
 
-SOURCES = ["ncs_solver.hip", "ncs_convnet.hip", "ncs_camera.hip", "ncs_capi.cpp"]
+SOURCES = ["ncs_solver.hip", "ncs_convnet.hip", "ncs_camera.hip", "ncs_subregion.hip",
+           "ncs_capi.cpp"]
 
 
 def hipcc():

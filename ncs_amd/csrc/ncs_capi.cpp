@@ -14,8 +14,8 @@
 #include "ncs_common.h"
 
 namespace ncs {
-int solve_waves_per_cu(int m, bool general, bool pad, int wpg);
-hipError_t launch_solve(int m, bool general, bool pad, unsigned int waves, int wpg,
+int solve_waves_per_cu(int m, bool general, bool pad, bool f16, int wpg);
+hipError_t launch_solve(int m, bool general, bool pad, bool f16, unsigned int waves, int wpg,
                         hipStream_t s, const SolveArgs &a);
 hipError_t launch_adu_to_photons(const uint16_t *adu, const float *offset, const float *gain,
                                  float *out, long long n, hipStream_t s);
@@ -49,6 +49,11 @@ static int fail(int code, const std::string &msg) {
   g_last_error = msg;
   return code;
 }
+
+namespace ncs {
+// For the library's other translation units (the sub-region ABI).
+void set_last_error(const char *msg) { g_last_error = msg; }
+}  // namespace ncs
 
 #define HIPCHK(x)                                                              \
   do {                                                                         \
@@ -85,12 +90,17 @@ struct DevBuf {
   template <class T> T *as() const { return reinterpret_cast<T *>(p); }
 };
 
-// A persistent-solve schedule: history depth, grid and workgroup size.
+// A persistent-solve schedule: history depth, grid, workgroup size and the
+// arithmetic of the basis changes.
 struct Schedule {
   int history = 5;
   int waves_per_cu = 0;  // 0: the occupancy limit
   int wpg = 1;
+  bool f16 = false;      // FP16-split basis changes (else FP32)
 };
+
+// Key of an autotuned schedule: mask kind, padded geometry, precision.
+typedef std::array<bool, 3> TuneKey;
 
 struct ncs_ctx {
   int device = 0;
@@ -106,8 +116,8 @@ struct ncs_ctx {
   std::vector<float> tab_host = std::vector<float>(kTableFloats, 0.0f);
   bool has_linear = false;
   float lin_c = 0.0f;
-  // autotuned schedules, keyed by (general mask, padded geometry)
-  std::map<std::pair<bool, bool>, Schedule> tuned;
+  // autotuned schedules, keyed by (general mask, padded geometry, FP16 split)
+  std::map<TuneKey, Schedule> tuned;
   // last uploaded frame descriptors (pinned staging + completion event)
   std::vector<FrameDev> last_frames;
   bool frames_valid = false;
@@ -140,6 +150,7 @@ extern "C" void ncs_hip_default_params(ncs_params *p) {
   p->waves_per_cu = 0;
   p->precond = 1;
   p->waves_per_group = 0;
+  p->precision = NCS_PRECISION_FP32;
 }
 
 extern "C" const char *ncs_hip_last_error(void) { return g_last_error.c_str(); }
@@ -305,7 +316,7 @@ extern "C" int ncs_hip_set_linear_start(ncs_ctx *ctx, const float *coef) {
 
 // Windows along an axis of n pixels: the reference geometry with side r
 // steps by s = r - 2 and needs n >= s; 1 + ceil((n - s) / s) windows
-// (ncs.c:71-143).
+// (ncs.c:71-143).  Blocks of side r: ceil(n / r).
 static long long tiles_axis(int n, int geom, int r = 16) {
   if (geom == NCS_GEOM_REFERENCE) {
     const int s = r - 2;
@@ -313,12 +324,12 @@ static long long tiles_axis(int n, int geom, int r = 16) {
     return 1 + (n - s + s - 1) / s;
   }
   if (n < 1) return -1;
-  return (n + 15) / 16;
+  return (n + r - 1) / r;
 }
 
 extern "C" long long ncs_hip_tile_count_window(int height, int width, int geometry, int r) {
   if (geometry != NCS_GEOM_REFERENCE && geometry != NCS_GEOM_BLOCKS) return -1;
-  if (r < 4 || r > 16 || (r & 1) || (geometry == NCS_GEOM_BLOCKS && r != 16)) return -1;
+  if (r < 4 || r > 16 || (r & 1)) return -1;
   long long a = tiles_axis(height, geometry, r), b = tiles_axis(width, geometry, r);
   if (a < 0 || b < 0) return -1;
   return a * b;
@@ -363,6 +374,8 @@ static int check_params(const ncs_params *p) {
     return fail(NCS_E_ARG, "waves_per_group must be 0..4");
   if (p->start < NCS_START_PLAIN || p->start > NCS_START_LINEAR)
     return fail(NCS_E_ARG, "unknown start");
+  if (p->precision != NCS_PRECISION_FP32 && p->precision != NCS_PRECISION_F16_SPLIT)
+    return fail(NCS_E_ARG, "unknown precision");
   return NCS_OK;
 }
 
@@ -375,9 +388,6 @@ static int build_frames(const ncs_ctx *ctx, const ncs_frame *frames, int n,
   for (int i = 0; i < n; i++) {
     const ncs_frame &f = frames[i];
     if (!f.data || !f.gamma) return fail(NCS_E_ARG, "frame without data or gamma");
-    if (win != 16 && f.geometry != NCS_GEOM_REFERENCE)
-      return fail(NCS_E_ARG, "frame " + std::to_string(i) + ": a " + std::to_string(win) +
-                  " x " + std::to_string(win) + " mask takes the reference geometry");
     long long nt = ncs_hip_tile_count_window(f.height, f.width, f.geometry, win);
     if (nt < 0)
       return fail(NCS_E_ARG, "frame " + std::to_string(i) + ": bad size or geometry (the "
@@ -449,17 +459,19 @@ static bool any_pad(const ncs_ctx *ctx, const ncs_frame *frames, int n) {
 // of this mask kind and geometry, then history 5, one wave per workgroup and
 // the occupancy limit.
 static Schedule resolve_schedule(const ncs_ctx *ctx, const ncs_params *p, bool pad) {
+  const bool f16 = p->precision == NCS_PRECISION_F16_SPLIT;
   Schedule d;
-  auto it = ctx->tuned.find({ctx->general, pad});
+  auto it = ctx->tuned.find(TuneKey{ctx->general, pad, f16});
   if (it != ctx->tuned.end()) d = it->second;
   Schedule s;
+  s.f16 = f16;
   s.history = p->history > 0 ? p->history : d.history;
   s.wpg = p->waves_per_group > 0 ? p->waves_per_group : d.wpg;
   // a tuned grid only applies to the depth and workgroup size it was tuned for
   const bool same = s.history == d.history && s.wpg == d.wpg;
   s.waves_per_cu = p->waves_per_cu > 0 ? p->waves_per_cu : (same ? d.waves_per_cu : 0);
   if (s.waves_per_cu <= 0) {
-    int w = solve_waves_per_cu(s.history, ctx->general, pad, s.wpg);
+    int w = solve_waves_per_cu(s.history, ctx->general, pad, s.f16, s.wpg);
     s.waves_per_cu = w > 0 ? w : 8;
   }
   return s;
@@ -502,7 +514,7 @@ static hipError_t launch_schedule(const ncs_ctx *ctx, const Schedule &sc, bool p
                                   long long total, hipStream_t s, const SolveArgs &a) {
   long long waves = (long long)sc.waves_per_cu * ctx->cus;
   waves = std::max<long long>(1, std::min(waves, total));
-  return launch_solve(sc.history, ctx->general, pad, (unsigned int)waves, sc.wpg, s, a);
+  return launch_solve(sc.history, ctx->general, pad, sc.f16, (unsigned int)waves, sc.wpg, s, a);
 }
 
 // Enqueue one persistent solve over device frames already described in
@@ -570,13 +582,15 @@ extern "C" int ncs_hip_solve_tiles_device(ncs_ctx *ctx, const float *data_in,
   if (rc) return rc;
   rc = check_window(ctx, params);
   if (rc) return rc;
+  const int r = ctx->win;  // tiles of the mask's side
+  if ((long long)r * num_tiles >= (1LL << 31) / r) return fail(NCS_E_ARG, "too many tiles");
   ncs_frame f;
   f.data = data_in;
   f.gamma = gamma;
   f.out = data_out;
-  f.height = 16 * num_tiles;
-  f.width = 16;
-  f.data_stride = f.gamma_stride = f.out_stride = 16;
+  f.height = r * num_tiles;
+  f.width = r;
+  f.data_stride = f.gamma_stride = f.out_stride = r;
   f.alpha = alpha;
   f.geometry = NCS_GEOM_BLOCKS;
   ncs_params p2 = *params;
@@ -675,13 +689,15 @@ extern "C" int ncs_hip_solve_tiles(ncs_ctx *ctx, const float *data_in, const flo
   if (!ctx || !data_in || !gamma || !data_out || num_tiles < 0)
     return fail(NCS_E_ARG, "bad arguments");
   if (num_tiles == 0) return NCS_OK;
+  const int r = ctx->win;  // tiles of the mask's side
+  if ((long long)r * num_tiles >= (1LL << 31) / r) return fail(NCS_E_ARG, "too many tiles");
   ncs_frame f;
   f.data = data_in;
   f.gamma = gamma;
   f.out = data_out;
-  f.height = 16 * num_tiles;
-  f.width = 16;
-  f.data_stride = f.gamma_stride = f.out_stride = 16;
+  f.height = r * num_tiles;
+  f.width = r;
+  f.data_stride = f.gamma_stride = f.out_stride = r;
   f.alpha = alpha;
   f.geometry = NCS_GEOM_BLOCKS;
   return ncs_hip_reduce_frames(ctx, &f, 1, params, iterations, status, cost);
@@ -910,10 +926,12 @@ extern "C" int ncs_hip_autotune(ncs_ctx *ctx, const ncs_frame *frames, int n_fra
     *ms = best;
     return r;
   };
+  const bool f16 = base->precision == NCS_PRECISION_F16_SPLIT;
   Schedule def;
   def.history = base->history > 0 ? base->history : 5;
   def.wpg = 1;
-  def.waves_per_cu = std::max(1, solve_waves_per_cu(def.history, ctx->general, pad, 1));
+  def.f16 = f16;
+  def.waves_per_cu = std::max(1, solve_waves_per_cu(def.history, ctx->general, pad, f16, 1));
   float ms_def = 0.0f;
   e = time_one(def, &ms_def);
   Schedule best = def;
@@ -926,7 +944,7 @@ extern "C" int ncs_hip_autotune(ncs_ctx *ctx, const ncs_frame *frames, int n_fra
   for (int m : depths) {
     for (int wpg : groups) {
       if (e != hipSuccess) break;
-      const int occ = solve_waves_per_cu(m, ctx->general, pad, wpg);
+      const int occ = solve_waves_per_cu(m, ctx->general, pad, f16, wpg);
       if (occ <= 0) continue;
       std::vector<int> grid = {occ, std::max(wpg, (occ * 3 / 4) / wpg * wpg),
                                std::max(wpg, (occ / 2) / wpg * wpg)};
@@ -937,6 +955,7 @@ extern "C" int ncs_hip_autotune(ncs_ctx *ctx, const ncs_frame *frames, int n_fra
         sc.history = m;
         sc.wpg = wpg;
         sc.waves_per_cu = w;
+        sc.f16 = f16;
         if (sc.history == def.history && sc.wpg == def.wpg && sc.waves_per_cu == def.waves_per_cu)
           continue;
         float ms = 0.0f;
@@ -954,7 +973,7 @@ extern "C" int ncs_hip_autotune(ncs_ctx *ctx, const ncs_frame *frames, int n_fra
   cleanup();
   HIPCHK(e);
   if (!tune_history) best.history = def.history;
-  ctx->tuned[{ctx->general, pad}] = best;
+  ctx->tuned[TuneKey{ctx->general, pad, f16}] = best;
   res->history = best.history;
   res->waves_per_cu = best.waves_per_cu;
   res->waves_per_group = best.wpg;
@@ -1058,9 +1077,6 @@ extern "C" int ncs_hip_pipeline_create(ncs_ctx *ctx, const ncs_pipeline_frame *f
   pl->off.resize(n_frames);
   for (int i = 0; i < n_frames; i++) {
     const ncs_pipeline_frame &f = frames[i];
-    if (ctx->win != 16 && f.geometry != NCS_GEOM_REFERENCE)
-      return fail(NCS_E_ARG, "a " + std::to_string(ctx->win) + " x " + std::to_string(ctx->win) +
-                  " mask takes the reference geometry");
     long long nt = ncs_hip_tile_count_window(f.height, f.width, f.geometry, ctx->win);
     if (nt < 0) return fail(NCS_E_ARG, "frame " + std::to_string(i) + ": bad size or geometry");
     if (!std::isfinite(f.alpha)) return fail(NCS_E_ARG, "non-finite alpha");

@@ -161,16 +161,26 @@ __device__ __forceinline__ float lane0(float x) {
 }
 
 // The solver's tables, read from LDS at each use (one copy per workgroup,
-// which saves 12-16 VGPRs per lane across the main loop): the mask weights,
-// the learned linear start's coefficients, and the basis entries as FP16
-// hi/lo pairs, each duplicated into slots r and 4 + r of an 8-wide operand.
+// which saves 12-16 VGPRs per lane across the main loop).  The FP16 build
+// (F16) holds the mask weights, the learned linear start's coefficients and
+// the basis entries as FP16 hi/lo pairs, each duplicated into slots r and
+// 4 + r of an 8-wide operand (table entries kSolveTableFirst..); the FP32
+// build holds the FP32 basis entries, the mask weights and the linear
+// start's coefficients (table entries 0..5).
+template <bool F16>
 struct LdsTables {
-  const f4 *p;  // this lane's kSolveTableF4 entries (kSolveTableFirst..)
-  __device__ f4 qe() const { return p[0]; }
-  __device__ f4 qo() const { return p[1]; }
-  __device__ f4 la() const { return p[2]; }
-  __device__ f4 lb() const { return p[3]; }
-  __device__ h8 kah() const { return __builtin_bit_cast(h8, p[4]); }
+  static constexpr bool kF16 = F16;
+  static constexpr int kFirst = F16 ? kSolveTableFirst : 0;  // first table entry held
+  static constexpr int kCount = F16 ? kSolveTableF4 : 6;     // entries held
+  static constexpr int o = F16 ? 0 : 2;
+  const f4 *p;  // this lane's kCount entries
+  __device__ f4 ka() const { return p[0]; }  // FP32 build only
+  __device__ f4 kb() const { return p[1]; }  // FP32 build only
+  __device__ f4 qe() const { return p[o + 0]; }
+  __device__ f4 qo() const { return p[o + 1]; }
+  __device__ f4 la() const { return p[o + 2]; }
+  __device__ f4 lb() const { return p[o + 3]; }
+  __device__ h8 kah() const { return __builtin_bit_cast(h8, p[4]); }  // FP16 build only
   __device__ h8 kal() const { return __builtin_bit_cast(h8, p[5]); }
   __device__ h8 kbh() const { return __builtin_bit_cast(h8, p[6]); }
   __device__ h8 kbl() const { return __builtin_bit_cast(h8, p[7]); }
@@ -286,12 +296,12 @@ __device__ __forceinline__ bool all_finite(const float (&z)[4]) {
 }
 
 // to_spec on the FP16 matrix cores: z = to_spec(x * s) / s (s a power of
-// two).  4 MFMA 16x16x32 f16; in FP32 when `exact` (uniform) is set.
-template <bool CHECK = true>
-__device__ __forceinline__ void to_spec16(const float (&x)[4], const LdsTables &T,
-                                          const GlobalBasis &G, float s, float (&z)[4],
-                                          bool exact) {
-  if (exact) {
+// two).  4 MFMA 16x16x32 f16; in FP32 (8 MFMA 16x16x4 f32, basis from G)
+// when `exact` (uniform) is set, and always in the FP32 build.
+template <bool CHECK = true, class TT, class GB>
+__device__ __forceinline__ void to_spec16(const float (&x)[4], const TT &T, const GB &G,
+                                          float s, float (&z)[4], bool exact) {
+  if (!TT::kF16 || exact) {
     to_spec(x, G, z);
     return;
   }
@@ -313,12 +323,11 @@ __device__ __forceinline__ void to_spec16(const float (&x)[4], const LdsTables &
 }
 
 // to_pix on the FP16 matrix cores: x = to_pix(w * s) / s.  4 MFMA; in
-// FP32 when `exact` is set.
-template <bool CHECK = true>
-__device__ __forceinline__ void to_pix16(const float (&w)[4], const LdsTables &T,
-                                         const GlobalBasis &G, float s, float (&x)[4],
-                                         bool exact) {
-  if (exact) {
+// FP32 when `exact` is set, and always in the FP32 build.
+template <bool CHECK = true, class TT, class GB>
+__device__ __forceinline__ void to_pix16(const float (&w)[4], const TT &T, const GB &G,
+                                         float s, float (&x)[4], bool exact) {
+  if (!TT::kF16 || exact) {
     to_pix(w, G, x);
     return;
   }
@@ -585,12 +594,15 @@ __device__ __forceinline__ TileGeom tile_geom(const FrameConst &F, long long loc
     G.c_lo = 1;
     G.c_hi = (G.by == lasty) ? r - 1 : min(r - 1, lasty + 1 - G.by);
   } else {
-    G.bx = 16 * a;
-    G.by = 16 * b;
+    // non-overlapping blocks of the window side r (16, or the side of an
+    // r x r mask) in the top-left of the tile, zero-padded past the frame
+    const int r = F.win;
+    G.bx = r * a;
+    G.by = r * b;
     G.r_lo = 0;
-    G.r_hi = min(16, F.H - G.bx);
+    G.r_hi = min(r, F.H - G.bx);
     G.c_lo = 0;
-    G.c_hi = min(16, F.W - G.by);
+    G.c_hi = min(r, F.W - G.by);
   }
   return G;
 }
@@ -680,7 +692,20 @@ __device__ __forceinline__ void ll_trial(float t, const float (&u)[4], const flo
 // 4, 4 waves (128 VGPRs) above; the latency of the reduction chains and of
 // the MFMA stages is hidden by other waves, so the extra wave pays
 // (measured: -10 % at history 1).
-template <int M, bool GEN, bool PAD>
+//
+// F16 selects the basis changes: FP16 hi/lo split on the FP16 matrix cores
+// (FP32 near the stop rule and as fallback), or FP32 MFMA throughout.
+template <bool F16> struct Basis32 { typedef GlobalBasis type; };
+template <> struct Basis32<false> { typedef LdsTables<false> type; };
+
+template <bool F16>
+__device__ __forceinline__ typename Basis32<F16>::type basis32(const LdsTables<F16> &T,
+                                                               const float *tables) {
+  if constexpr (F16) return GlobalBasis{tables};
+  else return T;
+}
+
+template <int M, bool GEN, bool PAD, bool F16>
 __global__ __launch_bounds__(64 * kMaxWavesPerGroup)
 __attribute__((amdgpu_waves_per_eu(M <= 4 ? 5 : 4, 8)))
 void solve_kernel(SolveArgs A) {
@@ -697,14 +722,15 @@ void solve_kernel(SolveArgs A) {
   // OTF / DFT tables: each wave writes its lanes' entries (the same values
   // for every wave of the group) and each lane reads back only its own, so
   // no barrier is needed.
-  __shared__ f4 tab_lds[kWave * kSolveTableF4];
+  typedef LdsTables<F16> TT;
+  __shared__ f4 tab_lds[kWave * TT::kCount];
   {
-    const f4 *tb = reinterpret_cast<const f4 *>(A.tables + lane * 4 * kTableF4) + kSolveTableFirst;
+    const f4 *tb = reinterpret_cast<const f4 *>(A.tables + lane * 4 * kTableF4) + TT::kFirst;
 #pragma unroll
-    for (int i = 0; i < kSolveTableF4; i++) tab_lds[lane * kSolveTableF4 + i] = tb[i];
+    for (int i = 0; i < TT::kCount; i++) tab_lds[lane * TT::kCount + i] = tb[i];
   }
-  const LdsTables T{tab_lds + lane * kSolveTableF4};
-  const GlobalBasis GB{A.tables};
+  const TT T{tab_lds + lane * TT::kCount};
+  const auto GB = basis32<F16>(T, A.tables);
 
   // Tile schedule.  Device-scope atomics are slow on this part (they are
   // resolved beyond the XCD's L2), so half the tiles are assigned statically:
@@ -841,7 +867,7 @@ void solve_kernel(SolveArgs A) {
     // gradient is within kF16Floor of the stop rule's scale, and in FP32
     // after that: a default-tolerance solve (eps 1e-5) stops before the
     // switch, a tighter one finishes with FP32 accuracy.
-    bool exact = false;
+    bool exact = !F16;
     {
       float gll[4];
       to_spec16<false>(u, T, GB, st, zs, exact);
@@ -1057,6 +1083,14 @@ void solve_kernel(SolveArgs A) {
         }
         wave_sum<1>(red);
         gg = red[0];
+        gnc_exact = true;
+        if (gg <= A.eps * A.eps * fmaxf(1.0f, uu)) {
+          // the exact gradient already meets the stop rule at u, the point
+          // the previous iteration accepted
+          iters = k - 1;
+          status = NCS_SUCCESS;
+          break;
+        }
         if (spec_h0) {
           h0_spec(ds, T, kappa, alpha);
           step = 1.0f;
@@ -1315,25 +1349,30 @@ __global__ void reset_queue_kernel(unsigned int *q) {
 }
 
 // Instantiations: history 1..8, with and without the odd-mask coupling and
-// zero-padded pixels.
+// zero-padded pixels, FP32 or FP16-split basis changes.
 typedef void (*SolveKernelFn)(SolveArgs);
 
-template <int M>
-static SolveKernelFn pick(bool general, bool pad) {
-  if (general) return pad ? solve_kernel<M, true, true> : solve_kernel<M, true, false>;
-  return pad ? solve_kernel<M, false, true> : solve_kernel<M, false, false>;
+template <int M, bool F16>
+static SolveKernelFn pick2(bool general, bool pad) {
+  if (general) return pad ? solve_kernel<M, true, true, F16> : solve_kernel<M, true, false, F16>;
+  return pad ? solve_kernel<M, false, true, F16> : solve_kernel<M, false, false, F16>;
 }
 
-static SolveKernelFn solve_kernel_for(int m, bool general, bool pad) {
+template <int M>
+static SolveKernelFn pick(bool general, bool pad, bool f16) {
+  return f16 ? pick2<M, true>(general, pad) : pick2<M, false>(general, pad);
+}
+
+static SolveKernelFn solve_kernel_for(int m, bool general, bool pad, bool f16) {
   switch (m) {
-    case 1: return pick<1>(general, pad);
-    case 2: return pick<2>(general, pad);
-    case 3: return pick<3>(general, pad);
-    case 4: return pick<4>(general, pad);
-    case 5: return pick<5>(general, pad);
-    case 6: return pick<6>(general, pad);
-    case 7: return pick<7>(general, pad);
-    case 8: return pick<8>(general, pad);
+    case 1: return pick<1>(general, pad, f16);
+    case 2: return pick<2>(general, pad, f16);
+    case 3: return pick<3>(general, pad, f16);
+    case 4: return pick<4>(general, pad, f16);
+    case 5: return pick<5>(general, pad, f16);
+    case 6: return pick<6>(general, pad, f16);
+    case 7: return pick<7>(general, pad, f16);
+    case 8: return pick<8>(general, pad, f16);
     default: return nullptr;
   }
 }
@@ -1344,8 +1383,8 @@ static size_t solve_lds_bytes(int m, int wpg) {
 
 // Waves of this instantiation one CU holds at once with workgroups of `wpg`
 // waves (0 if unsupported).
-int solve_waves_per_cu(int m, bool general, bool pad, int wpg) {
-  SolveKernelFn fn = solve_kernel_for(m, general, pad);
+int solve_waves_per_cu(int m, bool general, bool pad, bool f16, int wpg) {
+  SolveKernelFn fn = solve_kernel_for(m, general, pad, f16);
   if (!fn || wpg < 1 || wpg > kMaxWavesPerGroup) return 0;
   int blocks = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
@@ -1357,9 +1396,9 @@ int solve_waves_per_cu(int m, bool general, bool pad, int wpg) {
 
 // Resets the queue and launches `waves` persistent waves in workgroups of
 // `wpg` waves (waves is rounded up to a multiple of wpg).
-hipError_t launch_solve(int m, bool general, bool pad, unsigned int waves, int wpg,
+hipError_t launch_solve(int m, bool general, bool pad, bool f16, unsigned int waves, int wpg,
                         hipStream_t s, const SolveArgs &a) {
-  SolveKernelFn fn = solve_kernel_for(m, general, pad);
+  SolveKernelFn fn = solve_kernel_for(m, general, pad, f16);
   if (!fn || wpg < 1 || wpg > kMaxWavesPerGroup || waves == 0) return hipErrorInvalidValue;
   const unsigned int groups = (waves + wpg - 1) / wpg;
   hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(64), 0, s, a.queue);

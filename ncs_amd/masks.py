@@ -35,6 +35,31 @@ def otf_mask(r=16, pixel_size=0.1, NA=1.4, wavelength=0.7):
     return 1.0 - T
 
 
+def psf_otf(r=16, pixel_size=0.1, NA=1.4, wavelength=0.7):
+    """PSF, pupil and OTF of python3-6/pyNCS/denoisetools.py:103-115
+    (genpsfparam), centred r x r.
+
+    The pupil takes the value of the radial frequency |k| inside the
+    aperture |k| <= NA / lambda and 0 outside (the reference assigns the
+    frequency grid itself to the pupil, then zeroes it past the cutoff).
+    psfA = fftshift(fft2(pupil)), PSF = |psfA|^2 normalised to unit sum,
+    OTF = |fftshift(ifft2(|psfA|^2))| normalised to a maximum of 1: the
+    pupil's circular autocorrelation, with support up to 2 NA / lambda.
+    """
+    pupil = kspace(r, pixel_size)
+    pupil[pupil > NA / wavelength] = 0.0
+    psf_a = numpy.fft.fftshift(numpy.fft.fft2(pupil))
+    psf = (psf_a * psf_a.conj()).real
+    otf = numpy.abs(numpy.fft.fftshift(numpy.fft.ifft2(psf_a * psf_a.conj())))
+    return psf / psf.sum(), pupil, otf / otf.max()
+
+
+def psf_otf_mask(r=16, pixel_size=0.1, NA=1.4, wavelength=0.7):
+    """1 - the PSF-derived OTF of psf_otf (centred): 0 at DC, 1 past the
+    OTF's cutoff 2 NA / lambda."""
+    return 1.0 - psf_otf(r, pixel_size, NA, wavelength)[2]
+
+
 def gaussian_highpass_mask(r=16):
     """The Gaussian high-pass mask of opencl/pyOpenCLNCS/py_ref.py:129-137,
     returned centred (that function returns it in FFT order)."""

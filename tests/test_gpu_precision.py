@@ -1,7 +1,8 @@
 """GPU checks of the solver's precision machinery.
 
-The basis changes run on the FP16 matrix cores with every operand split
-into an FP16 hi/lo pair.  Data-like operands carry a per-tile power-of-two
+By default every basis change runs on FP32 MFMA.  With
+precision=PRECISION_F16_SPLIT they run on the FP16 matrix cores with every
+operand split into an FP16 hi/lo pair.  Data-like operands carry a per-tile power-of-two
 scale, a non-finite result falls back to FP32, and past a 1e-5 relative
 gradient the solver switches to FP32.  Data over nine decades of scale
 must still reach the FP64 oracle's minimiser, to the bars of
@@ -17,6 +18,9 @@ import oracle
 from ncs_amd import _lib, masks, sim
 
 pytestmark = pytest.mark.gpu
+
+F16 = _lib.PRECISION_F16_SPLIT
+FP32 = _lib.PRECISION_FP32
 
 
 def _say(*a):
@@ -49,12 +53,14 @@ def _check(ctx, d, g, params, label):
     assert numpy.all(gn <= 2e-5 * numpy.maximum(1.0, un)), label
 
 
+@pytest.mark.parametrize("precision", [FP32, F16])
 @pytest.mark.parametrize("scale", [1e-3, 1.0, 1e4, 1e6])
-def test_scaled_data_reach_the_oracle_minimiser(gpu_ctx, scale):
+def test_scaled_data_reach_the_oracle_minimiser(gpu_ctx, scale, precision):
     """Photon data times 1e-3 .. 1e6: the per-tile scale keeps the FP16
-    operands in range and precise at every magnitude."""
+    operands in range and precise at every magnitude; FP32 needs no scale."""
     d, g = _tiles(11, 64)
-    _check(gpu_ctx, d * scale, g * scale, _lib.default_params(), "scale %g" % scale)
+    _check(gpu_ctx, d * scale, g * scale, _lib.default_params(precision=precision),
+           "precision %d scale %g" % (precision, scale))
 
 
 def test_overflowing_gamma_transform_falls_back_to_fp32(gpu_ctx):
@@ -68,7 +74,8 @@ def test_overflowing_gamma_transform_falls_back_to_fp32(gpu_ctx):
     gpu_ctx.set_otf_mask(masks.to_fft_order(masks.otf_mask()).astype(numpy.float32))
     coef = numpy.random.default_rng(3).uniform(-0.3, 0.3, linstart.N_COEF).astype(numpy.float32)
     gpu_ctx.set_linear_start(coef)
-    p = _lib.default_params(start=_lib.START_LINEAR, max_iterations=0, fallback=0)
+    p = _lib.default_params(start=_lib.START_LINEAR, max_iterations=0, fallback=0,
+                            precision=F16)
     u, it, st, cost = gpu_ctx.solve_tiles(d32, g32, 0.1, p)
     ref = linstart.predict(d32, g32, coef)
     err = numpy.abs(u - ref).max(axis=(1, 2)) / numpy.abs(ref).max(axis=(1, 2))
@@ -93,7 +100,7 @@ def test_overflowing_gradient_falls_back_to_fp32(gpu_ctx):
     tu = torch.empty_like(td)
     ti = torch.empty(len(d), dtype=torch.int32, device=dev)
     ts = torch.empty_like(ti)
-    p = _lib.default_params(fallback=0, max_iterations=1)
+    p = _lib.default_params(fallback=0, max_iterations=1, precision=F16)
     gpu_ctx.solve_tiles_device(td.data_ptr(), tg.data_ptr(), tu0.data_ptr(), tu.data_ptr(),
                                ti.data_ptr(), ts.data_ptr(), None, len(d), 0.1, p,
                                torch.cuda.current_stream().cuda_stream)
@@ -114,7 +121,8 @@ def test_tight_tolerance_finishes_in_fp32(gpu_ctx):
     mf = masks.to_fft_order(masks.otf_mask()).astype(numpy.float32)
     gpu_ctx.set_otf_mask(mf)
     d32, g32 = d.astype(numpy.float32), g.astype(numpy.float32)
-    u, it, st, cost = gpu_ctx.solve_tiles(d32, g32, 0.1, _lib.default_params(epsilon=1e-7))
+    u, it, st, cost = gpu_ctx.solve_tiles(d32, g32, 0.1,
+                                          _lib.default_params(epsilon=1e-7, precision=F16))
     f64, gn, un = oracle.eval_cost_f32(u, d32, g32, mf, 0.1)
     _say("eps 1e-7: status", numpy.unique(st), "max |g|/max(1,|u|)",
          float(numpy.max(gn / numpy.maximum(1.0, un))))
@@ -132,7 +140,7 @@ def test_stalled_fp16_search_retries_in_fp32(gpu_ctx):
     fr = sim.make_frame(100, (96, 96))
     mask = masks.random_mask(r, numpy.random.default_rng(r))
     try:
-        h = ncs.NCSHip(size=r)
+        h = ncs.NCSHip(size=r, params=_lib.default_params(precision=F16))
         h.setOTFMask(mask)
         h.setGamma(fr["gamma"])
         out, info = h.reduceNoise([fr["data"]], 0.1, return_info=True)

@@ -49,6 +49,7 @@ def test_default_params():
     assert numpy.isclose(p.epsilon, 1e-5) and p.linesearch == _lib.LS_MORE_THUENTE
     assert numpy.isclose(p.ftol, 1e-4) and numpy.isclose(p.gtol, 0.9)
     assert p.precond == 1 and _lib.liblbfgs_params().precond == 0
+    assert p.precision == _lib.PRECISION_FP32  # FP32 basis changes unless asked
     k = _lib.kernel_semantics_params()
     assert k.history == 8 and k.linesearch == _lib.LS_ARMIJO and numpy.isclose(k.min_step, 1e-6)
 
@@ -105,5 +106,10 @@ def test_tile_counts_follow_the_window_side():
         for n in (r - 2, r - 1, r, 30, 31, 97, 128):
             assert _lib.tile_count(n, n, _lib.GEOM_REFERENCE, r) == loop(n, r) ** 2, (r, n)
         assert _lib.tile_count(r - 3, 40, _lib.GEOM_REFERENCE, r) == -1
-    assert _lib.tile_count(40, 40, _lib.GEOM_BLOCKS, 12) == -1
+    # blocks of side r: ceil(n / r) per axis
+    for r in (4, 10, 12, 16):
+        for n in (1, r - 1, r, r + 1, 40, 97):
+            assert _lib.tile_count(n, 40, _lib.GEOM_BLOCKS, r) == (-(-n // r)) * (-(-40 // r))
+    assert _lib.tile_count(0, 40, _lib.GEOM_BLOCKS, 12) == -1
+    assert _lib.tile_count(40, 40, _lib.GEOM_BLOCKS, 7) == -1
     assert _lib.tile_count(40, 40, _lib.GEOM_REFERENCE, 7) == -1

@@ -77,3 +77,37 @@ def test_photon_conversion_and_gamma():
     assert numpy.allclose(f["gamma"], g.astype(numpy.float32))
     f2 = sim.make_frame(5, (32, 40))
     assert numpy.array_equal(f["data"], f2["data"])     # seeded
+
+
+@pytest.mark.parametrize("r,pixel", [(16, 0.1), (12, 0.1), (16, 0.08)])
+def test_psf_otf_against_direct_sums(r, pixel):
+    """genpsfparam (denoisetools.py:103-115) by routes that use no FFT: the
+    PSF as |sum_x pupil(x) e^(-2 pi i k x / r)|^2 from an explicit DFT
+    matrix, the OTF as the pupil's circular autocorrelation summed over
+    shifts; both normalised as the reference does."""
+    psf, pupil, otf = masks.psf_otf(r=r, pixel_size=pixel)
+    kr = masks.kspace(r, pixel)
+    assert numpy.array_equal(pupil, numpy.where(kr > 1.4 / 0.7, 0.0, kr))
+    idx = numpy.arange(r)
+    F = numpy.exp(-2j * numpy.pi * numpy.outer(idx, idx) / r)
+    P = F @ pupil @ F.T
+    psf_direct = numpy.fft.fftshift(numpy.abs(P) ** 2)  # centring only reorders
+    assert numpy.allclose(psf, psf_direct / psf_direct.sum(), rtol=0, atol=1e-14)
+    ac = numpy.zeros((r, r))
+    for dy in range(r):
+        for dx in range(r):
+            ac[dy, dx] = numpy.sum(pupil * numpy.roll(numpy.roll(pupil, -dy, 0), -dx, 1))
+    ac = numpy.fft.fftshift(ac)  # zero shift to the centre
+    assert numpy.allclose(otf, ac / ac.max(), rtol=0, atol=1e-12)
+    assert otf[r // 2, r // 2] == 1.0 and abs(psf.sum() - 1.0) < 1e-12
+
+
+def test_psf_otf_mask_support_and_symmetry():
+    """1 - OTF: 0 at DC, 1 past the autocorrelation's support 2 NA / lambda,
+    point-symmetric (passes the reference's checkOTFMask)."""
+    m = masks.psf_otf_mask()
+    kr = masks.kspace(16, 0.1)
+    assert m[8, 8] == 0.0
+    assert numpy.allclose(m[kr > 2.0 * 1.4 / 0.7 + 1e-9], 1.0, atol=1e-12)
+    assert numpy.all((m >= -1e-12) & (m <= 1.0 + 1e-12))
+    assert ncs.checkOTFMask(m)
