@@ -3,30 +3,37 @@
 Workload (the plain-NCS batched configuration): 10 synthetic frames of
 2048 x 2048 float32 photo-electrons (WLC filament scenes, sCMOS camera maps,
 ncs_amd.sim), alpha = 0.1, the 1 - OTF mask of NA 1.4 / 700 nm / 100 nm
-pixels, L-BFGS history m = 5, at most 200 iterations, reference tiling
-(16 x 16 windows, stride 14: 21,609 tiles per frame, 216,090 per step).
+pixels, reference tiling (16 x 16 windows, stride 14: 21,609 tiles per
+frame, 216,090 per step), L-BFGS with history m = 5 as the reference run,
+at most 200 iterations, stop rule |g| <= 1e-5 max(1, |u|).
 
 A step is one pass of the hot path over the batch: one persistent launch
 that gathers every window from the frames in HBM, solves it and stitches the
 interiors back.  Inputs are resident in HBM before timing starts; the timed
 region is exactly --steps launches bracketed by torch.cuda.synchronize().
+Every timed path runs its basis changes on FP32 MFMA ("dtype": "f32"); the
+headline is the fastest path whose per-tile final costs match the m = 5
+plain run's within 1e-5 relative.  The FP16-split variant of the fastest
+path is reported beside it under its own dtype.
 
-Prints ONE JSON line (rank 0) with the throughput, the kernel's roofline
-figures (HIP events on the launch stream), the FP64 CPU oracle baseline on a
-bounded sample, and cost parity of a sample of tiles against the oracle.
-
-Two more workloads print their own line when asked for:
-  --workload uneven  one launch over 8 x 512^2, 4 x 1024x2048, 2 x 2304^2 and
-                     16 ROIs of 64-400 px per side, zero-padded 16 x 16 blocks,
-                     heavy-tailed variance, alpha per frame in [0.05, 0.3],
-                     autotuned schedule; device-resident and through the
-                     captured pipeline (host memory in and out);
-  --workload stream  200 frames of 2304^2 raw uint16 ADU (blinking filaments,
-                     background drifting 10 -> 40 photons/pixel, fixed camera
-                     maps) streamed frame by frame through two captured
-                     pipelines (ADU conversion, learned start, solve, stitch)
-                     with the host one frame ahead; sustained Mpix/s and
-                     per-frame latency (launch to consumption) p50 / p99.
+Prints ONE JSON line with the throughput, the headline kernel's roofline
+(issue cycles from the profiled SQ counters of the same solver sources,
+over the HIP-event kernel time), the FP64 CPU oracle baseline (all host
+threads of the process, at most 16, and one core), cost parity against
+the oracle, and two more workloads as keys:
+  uneven_batch   one launch over 8 x 512^2, 4 x 1024x2048, 2 x 2304^2 and
+                 16 ROIs of 64-400 px per side, zero-padded 16 x 16 blocks,
+                 heavy-tailed variance, alpha per frame in [0.05, 0.3],
+                 autotuned schedule; device-resident and through the
+                 captured pipeline (host memory in and out);
+  stream_series  200 frames of 2304^2 raw uint16 ADU (blinking filaments,
+                 background drifting 10 -> 40 photons/pixel, fixed camera
+                 maps) streamed frame by frame through two captured
+                 pipelines (ADU conversion, start, solve, stitch) with the
+                 host one frame ahead; sustained Mpix/s and per-frame latency
+                 (launch to consumption) p50 / p99.
+--workload uneven / stream print those as lines of their own; --only PATH
+launches one path of the batch (profiler runs).
 """
 import argparse
 import json
@@ -42,6 +49,8 @@ sys.path.insert(0, ROOT)
 METRIC = "corrected Mpix/s at matched NCS cost: learned-warm vs plain L-BFGS, 1 GPU"
 FP32_PEAK_TFLOPS = 157.3   # MI355X dense FP32 matrix peak
 HBM_PEAK_GBS = 8000.0
+SIMDS, CLOCK_GHZ = 256 * 4, 2.4  # MI355X: 256 CUs of 4 SIMDs, peak engine clock
+PCIE_GBS = 64.0  # PCIe 5.0 x16, one direction
 N_FRAMES, H, W, ALPHA, HISTORY, MAX_IT = 10, 2048, 2048, 0.1, 5, 200
 
 
@@ -73,20 +82,44 @@ def _make_frames(n, pool_size):
     return [_frame(s) for s in range(1, n + 1)]
 
 
-def profiled(key, kernel, path):
-    """A per-launch figure of `kernel` run on `path` (plain, tuned, linear)
-    for this workload from the committed rocprofv3 summaries, or None:
-    "traffic_bytes_per_launch" (FETCH_SIZE and WRITE_SIZE passes calibrated
-    on a device copy) or "mfma_instructions_per_launch" (SQ_INSTS_MFMA)."""
+SOLVER_SOURCES = ("ncs_amd/csrc/ncs_solver.hip", "ncs_amd/csrc/ncs_common.h", "include/ncs_hip.h")
+
+
+def solver_source_hash():
+    """First 12 hex digits of the SHA-256 of the solver's sources: the
+    profiled per-launch figures below apply only to the code they were
+    measured on."""
+    import hashlib
+    h = hashlib.sha256()
+    for rel in SOLVER_SOURCES:
+        with open(os.path.join(ROOT, rel), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:12]
+
+
+def profiled(kind, kernel, path, precision):
+    """Per-launch figures of `kernel` on `path` (plain, tuned, linear, ...)
+    and `precision` (fp32, f16split) from the rocprofv3 summaries in
+    profiles/, as a dict of numbers, or None when there are none for the
+    current solver sources.  Lines read
+
+        <kind> <kernel> <path> <precision> src=<hash>: key=value ...
+
+    kind "sq_per_launch" (SQ_INSTS_VALU, SQ_INSTS_MFMA,
+    SQ_INSTS_VALU_TRANS_F32, SQ_INSTS_SALU, SQ_WAVES) or "traffic_per_launch"
+    (FETCH_SIZE and WRITE_SIZE passes with the copy calibration)."""
     import re
     d = os.path.join(ROOT, "profiles")
-    pat = re.compile(r"^%s (\S+) (\S+): (\d+)$" % re.escape(key))
+    src = solver_source_hash()
+    pat = re.compile(r"^%s (\S+) (\S+) (\S+) src=([0-9a-f]+): (.*)$" % re.escape(kind))
     for fn in (sorted(os.listdir(d)) if os.path.isdir(d) else []):
         with open(os.path.join(d, fn)) as fh:
             for line in fh:
                 m = pat.match(line.strip())
-                if m and m.group(1) == kernel and m.group(2) == path:
-                    return int(m.group(3))
+                if (m and m.group(1) == kernel and m.group(2) == path
+                        and m.group(3) == precision and m.group(4) == src):
+                    return {k: float(v) for k, v in
+                            (kv.split("=") for kv in m.group(5).split())}
     return None
 
 
@@ -109,6 +142,78 @@ def flops_per_tile(iters, trials, m, warm=False):
     # a warm start scores both starts: one more evaluation
     return (f_eval * (1.0 + trials + (1.0 if warm else 0.0)) + 512.0 * trials
             + (13.0 + 8.0 * m) * 256.0 * iters)
+
+
+def issue_cycles(sq, f16):
+    """SIMD issue cycles per launch from the profiled instruction counts.
+
+    A wave64 VALU instruction occupies its SIMD's vector datapath for 4
+    cycles (16 lanes per cycle), a transcendental one 16 (quarter rate).  An
+    FP32 MFMA 16x16x4 (2,048 flop) runs on the same datapath for 32 cycles
+    (64 flop per cycle per SIMD: the 157.3 TFLOP/s FP32 matrix peak), so in
+    the FP32 build the two add.  The FP16 build's MFMA 16x16x32 run on the
+    matrix cores beside the VALU; there only the VALU is counted."""
+    valu, mfma, trans = sq["valu"], sq["mfma"], sq.get("trans", 0.0)
+    cycles = 4.0 * (valu - mfma) + 12.0 * trans
+    if not f16:
+        cycles += 32.0 * mfma
+    return cycles
+
+
+def roofline_of(kms, iters, m, path, precision, pix, warm=False, pad=False):
+    """Roofline figures of one solve launch (kernel time kms, per-tile
+    iterations `iters`) on `path` / `precision` ("fp32" or "f16split").
+
+    The binding unit is the SIMD's issue: `achieved` is the profiled issue
+    cycles per launch (issue_cycles) over the launch time, `peak` the 1,024
+    SIMDs at the 2.4 GHz peak clock, so frac <= 1 is the share of the
+    launch the datapath is busy.  The algorithmic work of the DFT-matrix
+    formulation (flops_per_tile) is reported beside it, as are HBM bytes:
+    algorithmic (data and gamma in, image out, 12 B per tile of results)
+    and measured (profiled FETCH_SIZE / WRITE_SIZE, calibrated)."""
+    f16 = precision == "f16split"
+    kernel = "solve_kernel<%d,false,%s,%s>" % (m, "true" if pad else "false",
+                                               "true" if f16 else "false")
+    t = kms * 1e-3
+    it = numpy.asarray(iters, dtype=numpy.float64)
+    ntiles = len(it)
+    flops = float(numpy.sum(flops_per_tile(it, it, m, warm)))
+    alg_bytes = pix * 12.0 + ntiles * 12.0
+    peak = SIMDS * CLOCK_GHZ
+    sq = profiled("sq_per_launch", kernel, path, precision)
+    tr = profiled("traffic_per_launch", kernel, path, precision)
+    out = {"bound": "valu", "achieved": None, "peak": round(peak, 1), "unit": "G issue-cycles/s",
+           "frac": None, "traffic": int(tr["bytes"]) if tr else None,
+           "kernel": kernel, "kernel_ms": round(kms, 4), "precision": precision,
+           "model": "issue cycles = 4 x VALU (non-MFMA) + 12 x transcendental + "
+                    + ("0 x MFMA (FP16, matrix cores)" if f16 else "32 x MFMA 16x16x4 f32")
+                    + " per launch (SQ counters, profiles/), over SIMDs x 2.4 GHz x launch time",
+           "algorithmic_tflops": round(flops / t / 1e12, 3),
+           "algorithmic_flop_per_launch": flops,
+           "algorithmic_flop_model": "F_eval (1 + T) + 512 T + (13 + 8m) 256 I per tile, "
+                                     "F_eval = 102,656 (DFT-matrix evaluation), T = I",
+           "hbm_gbs_algorithmic": round(alg_bytes / t / 1e9, 2),
+           "hbm_bytes_algorithmic": alg_bytes}
+    if sq:
+        cyc = issue_cycles(sq, f16)
+        ach = cyc / t / 1e9
+        out.update(achieved=round(ach, 2), frac=round(ach / peak, 4),
+                   issue_cycles_per_launch=cyc,
+                   per_tile={"valu": round(sq["valu"] / ntiles, 1),
+                             "mfma": round(sq["mfma"] / ntiles, 2),
+                             "trans": round(sq.get("trans", 0.0) / ntiles, 1),
+                             "salu": round(sq.get("salu", 0.0) / ntiles, 1)})
+        if f16:
+            out["executed_mfma_tflops_f16"] = round(sq["mfma"] * 16384.0 / t / 1e12, 2)
+        else:
+            out["executed_mfma_tflops_f32"] = round(sq["mfma"] * 2048.0 / t / 1e12, 2)
+    else:
+        out["profile"] = ("no SQ profile of %s on path %s for solver sources %s in profiles/"
+                          % (kernel, path, solver_source_hash()))
+    if tr:
+        out["hbm_gbs_measured"] = round(tr["bytes"] / t / 1e9, 2)
+        out["traffic_unit"] = "bytes per launch (FETCH_SIZE, WRITE_SIZE, copy-calibrated)"
+    return out
 
 
 def run_gpu(frames_np, args, weights, lin_coef=None):
@@ -134,6 +239,16 @@ def run_gpu(frames_np, args, weights, lin_coef=None):
     results = {}
 
     def timed(params, label, steps, warmup):
+        if args.only and label != args.only:
+            # a profiler run of one path: the others are not launched; the
+            # plain run's results stand in for the parity checks that need them
+            if label == "plain":
+                results[label] = dict(wall_s=float("nan"), kernel_ms=float("nan"),
+                                      iters=numpy.zeros(total, numpy.int32),
+                                      status=numpy.zeros(total, numpy.int32),
+                                      cost=numpy.ones(total, numpy.float32), out=None,
+                                      steps=0)
+            return
         # The timed launches return the image, iterations and status, as
         # the reference's reduceNoise returns the image; the per-tile costs
         # the parity checks use come from one more launch after the timing
@@ -217,9 +332,30 @@ def run_gpu(frames_np, args, weights, lin_coef=None):
                                   waves_per_group=tune_l["waves_per_group"],
                                   start=_lib.START_LINEAR),
               "linear", args.steps, args.warmup)
+    tune_f = None
+    if lin_coef is not None and not args.no_f16:
+        # the linear path with the FP16-split basis changes (4 MFMA 16x16x32
+        # f16 on hi/lo operand halves instead of 8 FP32 MFMA 16x16x4), tuned
+        # for itself
+        F16 = _lib.PRECISION_F16_SPLIT
+        if sched is None or sched.get("f16") is None:
+            tune_f = ctx.autotune(desc, _lib.default_params(max_iterations=MAX_IT,
+                                                            start=_lib.START_LINEAR,
+                                                            precision=F16),
+                                  tune_history=True)
+        else:
+            tune_f = sched["f16"]
+        print("[bench] autotune of the FP16-split linear path: %s" % tune_f, file=sys.stderr,
+              flush=True)
+        results["tune_f16"] = tune_f
+        timed(_lib.default_params(history=tune_f["history"], max_iterations=MAX_IT,
+                                  waves_per_cu=tune_f["waves_per_cu"],
+                                  waves_per_group=tune_f["waves_per_group"],
+                                  start=_lib.START_LINEAR, precision=F16),
+              "f16_split", args.steps, args.warmup)
     if sched is None and args.schedule_cache:
         with open(args.schedule_cache, "w") as fh:
-            json.dump({"m5": tune, "tuned": tune_h, "linear": tune_l}, fh)
+            json.dump({"m5": tune, "tuned": tune_h, "linear": tune_l, "f16": tune_f}, fh)
     if weights is not None:
         from ncs_amd import warmstart
         warmstart.set_context_weights(weights)
@@ -228,6 +364,7 @@ def run_gpu(frames_np, args, weights, lin_coef=None):
                                       waves_per_group=tune_h["waves_per_group"],
                                       start=_lib.START_LEARNED)
         timed(learned, "learned", args.steps, args.warmup)
+    if weights is not None and (not args.only or args.only == "learned"):
         # the net alone, and the fallback decision on the first frame
         u0 = torch.empty((ntile_frame * len(d), 256), dtype=torch.float32, device=dev)
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -273,42 +410,86 @@ def parity_sample(frames_np, res, ntile_frame, n_sample_frames=1):
                 max_abs_pixel=dmax, within_1e5=bool(rel.max() <= 1e-5)), el
 
 
+def host_threads():
+    """(threads, visible): the CPUs this process may run on, and the worker
+    threads the baseline uses, at most 16 (one GPU's share of the host's
+    CPUs on the benchmark machines)."""
+    try:
+        visible = len(os.sched_getaffinity(0))
+    except AttributeError:
+        visible = os.cpu_count() or 1
+    return max(1, min(visible, 16)), visible
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or "unknown"
+
+
 def cpu_baseline(frames_np, budget_s):
-    """FP64 oracle (C restatement of clib/pyCNCS/ncs.c) on all host cores,
-    same parameters, whole-image path; a bounded number of frames."""
+    """The FP64 oracle (C restatement of clib/pyCNCS/ncs.c, liblbfgs's
+    algorithm, More-Thuente) on the host, same parameters, whole-image path
+    (tiling, solve, stitch): on every thread the baseline may use, over as
+    many frames as fit budget_s, and on one core over a 1024 x 1024 crop of
+    the first frame."""
     import oracle
     from ncs_amd import masks
     mf = masks.to_fft_order(masks.otf_mask())
     p = oracle.default_param(m=HISTORY, max_iterations=MAX_IT)
-    cores = min(os.cpu_count() or 1, 16)
+    threads, visible = host_threads()
     n, t = 0, 0.0
     while n < len(frames_np) and t < budget_s:
         t0 = time.perf_counter()
         oracle.reduce_noise(frames_np[n][0].astype(numpy.float64),
                             frames_np[n][1].astype(numpy.float64), mf, ALPHA, param=p,
-                            nthreads=cores)
+                            nthreads=threads)
         t += time.perf_counter() - t0
         n += 1
-    return dict(value=n * H * W / t / 1e6, unit="Mpix/s", cores=cores, kind="port",
+    crop = (frames_np[0][0][:1024, :1024].astype(numpy.float64),
+            frames_np[0][1][:1024, :1024].astype(numpy.float64))
+    t0 = time.perf_counter()
+    oracle.reduce_noise(crop[0], crop[1], mf, ALPHA, param=p, nthreads=1)
+    t1 = time.perf_counter() - t0
+    return dict(value=n * H * W / t / 1e6, unit="Mpix/s", cores=threads, kind="port",
                 sample="%d frame(s) of %dx%d, reference tiling, m=%d, eps=1e-5, "
-                       "More-Thuente, FP64, OpenMP over tiles" % (n, H, W, HISTORY))
+                       "More-Thuente, FP64, OpenMP over tiles, %d threads"
+                       % (n, H, W, HISTORY, threads),
+                cpu_model=cpu_model(), host_cpus_visible=visible,
+                one_core={"value": 1024 * 1024 / t1 / 1e6, "unit": "Mpix/s", "cores": 1,
+                          "sample": "1024x1024 crop of frame 1, same parameters"})
 
 
 # ---------------------------------------------------------------------------
 # uneven batch: mixed sizes and ROIs, zero-padded blocks, per-frame alpha
 # ---------------------------------------------------------------------------
 
-def run_uneven(args):
-    import torch
-    from ncs_amd import _lib, masks
-    from ncs_amd.workloads import block_tiles, uneven_frame, uneven_specs
+def make_uneven_frames():
+    """The uneven stack's frames, generated by a process pool (before the GPU
+    is initialised)."""
+    from ncs_amd.workloads import uneven_frame, uneven_specs
     shapes, alphas, seeds = uneven_specs()
     t0 = time.perf_counter()
     from multiprocessing import get_context
-    with get_context("fork").Pool(min(16, os.cpu_count() or 1)) as pool:
+    with get_context("fork").Pool(host_threads()[0]) as pool:
         frames = pool.map(uneven_frame, list(zip(seeds, shapes)))
     print("[bench] uneven: generated %d frames in %.1f s" % (len(frames), time.perf_counter() - t0),
           file=sys.stderr, flush=True)
+    return frames
+
+
+def run_uneven(args, frames):
+    """The uneven batch; returns its JSON object."""
+    import torch
+    from ncs_amd import _lib, masks
+    from ncs_amd.workloads import block_tiles, uneven_specs
+    shapes, alphas, seeds = uneven_specs()
     dev = torch.device("cuda:0")
     ctx = _lib.context(0)
     mf = masks.to_fft_order(masks.otf_mask()).astype(numpy.float32)
@@ -418,7 +599,7 @@ def run_uneven(args):
         ref = oracle.solve_tiles_masked(dt, gt, m, mf, alphas[i],
                                         param=oracle.default_param(m=tune["history"],
                                                                    max_iterations=MAX_IT),
-                                        nthreads=min(16, os.cpu_count() or 1))
+                                        nthreads=host_threads()[0])
         c = gc[firsts[i]:firsts[i + 1]]
         rel.append(numpy.abs(c - ref["cost"]) / numpy.abs(ref["cost"]))
     rel = numpy.concatenate(rel)
@@ -428,6 +609,11 @@ def run_uneven(args):
     best_wall, best_path = wall, "plain"
     if lin is not None and lin["matched_1e5"] and lwall < wall:
         best_wall, best_path = lwall, "linear"
+    if best_path == "linear":
+        roof = roofline_of(lkms, lit, tune_l["history"], "uneven_linear", "fp32", pix,
+                           warm=True, pad=True)
+    else:
+        roof = roofline_of(kms, it, tune["history"], "uneven_plain", "fp32", pix, pad=True)
     line = {
         "metric": "corrected Mpix/s, uneven batch in one launch, 1 GPU",
         "value": round(pix / best_wall / 1e6, 2), "unit": "Mpix/s", "n_gpus": 1,
@@ -454,8 +640,9 @@ def run_uneven(args):
                    "p99_rel_cost": float(numpy.percentile(rel, 99)),
                    "within_1e5": bool(rel.max() <= 1e-5),
                    "oracle": "FP64 masked-likelihood oracle, same history"},
+        "roofline": roof,
     }
-    print(json.dumps(line), flush=True)
+    return line
 
 
 # ---------------------------------------------------------------------------
@@ -572,9 +759,25 @@ def run_stream(args, weights, lin_coef=None):
             e["max_abs_pixel"] = max(e["max_abs_pixel"],
                                      float(numpy.max(numpy.abs(r["kept"][k] - o))))
             e["bit_exact_plain"] = e["bit_exact_plain"] and bool(numpy.array_equal(r["kept"][k], o))
+    # and the first frame against the FP64 oracle, every tile
+    import oracle
+    ph = (adu[0].astype(numpy.float32) - off32) / gain32
+    ph[ph <= 0] = numpy.float32(1e-6)
+    _, oinfo = oracle.reduce_noise(ph.astype(numpy.float64),
+                                   (var32 / (gain32 * gain32)).astype(numpy.float64),
+                                   mf.astype(numpy.float64), ALPHA,
+                                   param=oracle.default_param(m=HISTORY, max_iterations=MAX_IT),
+                                   nthreads=host_threads()[0])
+    for label in results:
+        rel = (numpy.abs(results[label]["costs"][0].astype(numpy.float64) - oinfo["cost"])
+               / numpy.abs(oinfo["cost"]))
+        par[label]["oracle_frame0"] = {"tiles": int(rel.size), "max_rel_cost": float(rel.max()),
+                                       "p99_rel_cost": float(numpy.percentile(rel, 99)),
+                                       "within_1e5": bool(rel.max() <= 1e-5)}
     best = "plain"
     for label in results:
-        if par[label]["max_rel_cost"] <= 1e-5 and results[label]["wall"] < results[best]["wall"]:
+        if (par[label]["max_rel_cost"] <= 1e-5 and par[label]["oracle_frame0"]["within_1e5"]
+                and results[label]["wall"] < results[best]["wall"]):
             best = label
     pix = n * shape[0] * shape[1]
 
@@ -604,8 +807,17 @@ def run_stream(args, weights, lin_coef=None):
         "latency_ms_p50": summary(best)["latency_ms_p50"],
         "latency_ms_p99": summary(best)["latency_ms_p99"],
         "modes": {label: summary(label) for label in results},
+        "parity": par[best],
+        # host to host: every frame crosses PCIe twice, 2 B/px up (uint16 ADU)
+        # and 4 B/px down (float32 image); the download is the larger
+        "roofline": {"bound": "pcie", "unit": "GB/s", "peak": PCIE_GBS,
+                     "achieved": round(pix * 4.0 / results[best]["wall"] / 1e9, 2),
+                     "frac": round(pix * 4.0 / results[best]["wall"] / 1e9 / PCIE_GBS, 4),
+                     "direction": "device to host (float32 corrected frames)",
+                     "host_to_device_gbs": round(pix * 2.0 / results[best]["wall"] / 1e9, 2),
+                     "traffic": None},
     }
-    print(json.dumps(line), flush=True)
+    return line
 
 
 def _scalar_summary(res, plain, pix):
@@ -632,6 +844,15 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-learned", action="store_true", help="skip the conv-net warm start")
     ap.add_argument("--no-linear", action="store_true", help="skip the learned linear start")
+    ap.add_argument("--no-f16", action="store_true",
+                    help="skip the FP16-split run of the linear path")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="skip the uneven and stream workloads of the default run")
+    ap.add_argument("--only", default=None,
+                    choices=("plain", "plain_scalar_h0", "tuned", "linear", "f16_split",
+                             "learned", "fixed200"),
+                    help="launch only this path of the batch workload and print nothing "
+                         "but its timing (profiler runs); implies --batch-only --no-cpu")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a 1 GiB device copy first (PMC byte-counter calibration)")
@@ -654,7 +875,7 @@ def main():
         sys.exit(2)
 
     if args.workload == "uneven":
-        run_uneven(args)
+        print(json.dumps(run_uneven(args, make_uneven_frames())), flush=True)
         return
     if args.workload == "stream":
         weights = None
@@ -667,8 +888,11 @@ def main():
             from ncs_amd import linstart
             if os.path.exists(linstart.DEFAULT_COEF):
                 lin_coef = linstart.load()
-        run_stream(args, weights, lin_coef)
+        print(json.dumps(run_stream(args, weights, lin_coef)), flush=True)
         return
+    if args.only:
+        args.batch_only = args.no_cpu = True
+    uneven_frames = None if args.batch_only else make_uneven_frames()
 
     # learned warm start weights (trained on the GPU by ncs_amd.warmstart);
     # process pools run before the GPU is initialised
@@ -712,45 +936,35 @@ def main():
             lin_note = ("ncs_amd/weights/%s (fitted in closed form on MI355X solver pairs)"
                         % os.path.basename(linstart.DEFAULT_COEF))
     res, ntile_frame, total, devinfo = run_gpu(frames_np, args, weights, lin_coef)
+    if args.only:
+        r = res[args.only]
+        print(json.dumps({"path": args.only, "kernel_ms": r["kernel_ms"],
+                          "ms_per_step": r["wall_s"] * 1e3, "steps": r["steps"],
+                          "iterations_mean": float(r["iters"].mean()),
+                          "source_hash": solver_source_hash()}), flush=True)
+        return
     plain = res["plain"]
     pix = len(frames_np) * H * W
     mpix = pix / plain["wall_s"] / 1e6
-    alg_bytes = pix * 12.0  # data + gamma in, image out (float32)
     it = plain["iters"].astype(numpy.int64)
 
-    def roofline_of(run, m, path, warm=False):
-        kms = run["kernel_ms"]
-        it = run["iters"].astype(numpy.float64)
-        # trials T = I: one trial per iteration, a lower bound (the first
-        # trial is accepted on almost every iteration)
-        flops = float(numpy.sum(flops_per_tile(it, it, m, warm)))
-        achieved = flops / (kms * 1e-3) / 1e12
-        kernel = "solve_kernel<%d,false,false>" % m
-        return {"bound": "mfma", "achieved": round(achieved, 3),
-                "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
-                "traffic": profiled("traffic_bytes_per_launch", kernel, path),
-                "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0, "
-                                "profiles/)",
-                "flop_model": "algorithmic: F_eval (1 + T) + 512 T + (13 + 8m) 256 I per tile, "
-                              "F_eval = 102,656 (DFT-matrix evaluation), T = I",
-                "kernel": kernel, "kernel_ms": round(kms, 3),
-                # the basis changes run MFMA 16x16x32 f16 (16,384 flops each)
-                "executed_mfma_tflops_f16": (
-                    round(profiled("mfma_instructions_per_launch", kernel, path) * 16384.0
-                          / (kms * 1e-3) / 1e12, 2)
-                    if profiled("mfma_instructions_per_launch", kernel, path) else None),
-                "flop_per_launch": flops,
-                "hbm_gbs_algorithmic": round(alg_bytes / (kms * 1e-3) / 1e9, 2)}
+    def vs_plain(run):
+        rel = (numpy.abs(run["cost"].astype(numpy.float64) - plain["cost"])
+               / numpy.abs(plain["cost"]))
+        return rel
+
+    def status_of(run):
+        return {str(k): int(v) for k, v in zip(*numpy.unique(run["status"], return_counts=True))}
 
     parity, _ = parity_sample(frames_np, plain, ntile_frame, 1)
-    value, start_used = mpix, "plain"
-    candidates = {"plain": (mpix, True, plain["wall_s"])}
+    # FP32 paths that may carry the headline: (Mpix/s, matched, wall, history,
+    # path key, warm start)
+    candidates = {"plain": (mpix, True, plain["wall_s"], HISTORY, "plain", False)}
     tuned_info = None
     if "tuned" in res:
         T = res["tuned"]
         tm = pix / T["wall_s"] / 1e6
-        rel = numpy.abs(T["cost"].astype(numpy.float64) - plain["cost"]) / numpy.abs(plain["cost"])
+        rel = vs_plain(T)
         tuned_info = {"value": round(tm, 2), "unit": "Mpix/s",
                       "ms_per_step": round(T["wall_s"] * 1e3, 3),
                       "kernel_ms": round(T["kernel_ms"], 3),
@@ -760,12 +974,13 @@ def main():
                       "iterations_mean": round(float(T["iters"].mean()), 3),
                       "cost_vs_plain_max_rel": float(rel.max()),
                       "matched_1e5": bool(rel.max() <= 1e-5)}
-        candidates["tuned"] = (tm, tuned_info["matched_1e5"], T["wall_s"])
+        candidates["tuned"] = (tm, tuned_info["matched_1e5"], T["wall_s"],
+                               res["tune_history"]["history"], "tuned", False)
     linear_info = None
     if "linear" in res:
         L = res["linear"]
         lm = pix / L["wall_s"] / 1e6
-        rel = numpy.abs(L["cost"].astype(numpy.float64) - plain["cost"]) / numpy.abs(plain["cost"])
+        rel = vs_plain(L)
         TL = res["tune_linear"]
         linear_info = {
             "value": round(lm, 2), "unit": "Mpix/s", "ms_per_step": round(L["wall_s"] * 1e3, 3),
@@ -778,44 +993,65 @@ def main():
             "cost_vs_plain_p99_rel": float(numpy.percentile(rel, 99)),
             "matched_1e5": bool(rel.max() <= 1e-5),
             "coefficients": lin_note,
-            "status": {str(k): int(v) for k, v in zip(*numpy.unique(L["status"], return_counts=True))}}
-        candidates["linear"] = (lm, linear_info["matched_1e5"], L["wall_s"])
+            "status": status_of(L)}
+        candidates["linear"] = (lm, linear_info["matched_1e5"], L["wall_s"], TL["history"],
+                                "linear", True)
+    f16_info = None
+    if "f16_split" in res:
+        F = res["f16_split"]
+        fm = pix / F["wall_s"] / 1e6
+        rel = vs_plain(F)
+        TF = res["tune_f16"]
+        f16_info = {
+            "value": round(fm, 2), "unit": "Mpix/s", "dtype": "f16-split/f32-acc",
+            "note": "the linear path with every FP32 operand of the basis changes split into "
+                    "two FP16 halves on the FP16 matrix cores, FP32 accumulation; FP32 MFMA "
+                    "near the stop rule and on overflow",
+            "ms_per_step": round(F["wall_s"] * 1e3, 3), "kernel_ms": round(F["kernel_ms"], 3),
+            "history": TF["history"], "waves_per_cu": TF["waves_per_cu"],
+            "waves_per_group": TF["waves_per_group"],
+            "iterations_mean": round(float(F["iters"].mean()), 3),
+            "cost_vs_plain_max_rel": float(rel.max()),
+            "matched_1e5": bool(rel.max() <= 1e-5),
+            "speedup_vs_fp32_linear": (round(fm / candidates["linear"][0], 3)
+                                       if "linear" in candidates else None),
+            "status": status_of(F),
+            "roofline": roofline_of(F["kernel_ms"], F["iters"], TF["history"], "linear",
+                                    "f16split", pix, warm=True)}
     learned_info = None
     if "learned" in res:
         L = res["learned"]
         lm = pix / L["wall_s"] / 1e6
-        rel = numpy.abs(L["cost"].astype(numpy.float64) - plain["cost"]) / numpy.abs(plain["cost"])
-        matched = bool(rel.max() <= 1e-5)
+        rel = vs_plain(L)
         learned_info = {
             "value": round(lm, 2), "unit": "Mpix/s", "ms_per_step": round(L["wall_s"] * 1e3, 3),
             "kernel_ms_net_plus_solve": round(L["kernel_ms"], 3),
             "net_ms": round(L.get("net_ms", float("nan")), 3),
+            "net_tflops_bf16": round(len(plain["iters"]) * 9879552.0
+                                     / (L.get("net_ms", float("nan")) * 1e-3) / 1e12, 1),
+            "net_frac_of_bf16_peak": round(len(plain["iters"]) * 9879552.0
+                                           / (L.get("net_ms", float("nan")) * 1e-3)
+                                           / 2.5e15, 4),
             "iterations_mean": round(float(L["iters"].mean()), 3),
             "iterations_saved": round(float(plain["iters"].mean() - L["iters"].mean()), 3),
             "fallback_rate_frame1": round(L.get("fallback_rate", float("nan")), 4),
             "median_start_cost_gain_rel": L.get("start_gain_rel"),
             "cost_vs_plain_max_rel": float(rel.max()),
             "cost_vs_plain_p99_rel": float(numpy.percentile(rel, 99)),
-            "matched_1e5": matched,
+            "matched_1e5": bool(rel.max() <= 1e-5),
             "speedup_vs_plain": round(lm / mpix, 3),
             "speedup_vs_plain_scalar_h0": (round(lm * res["plain_scalar_h0"]["wall_s"] / pix * 1e6, 3)
                                            if "plain_scalar_h0" in res else None),
             "weights": weights_note,
-            "status": {str(k): int(v) for k, v in zip(*numpy.unique(L["status"], return_counts=True))}}
-        candidates["learned"] = (lm, matched, L["wall_s"])
-    # headline: the fastest path whose per-tile costs match the plain run's
-    # (m = 5, spectral H0) within 1e-5
-    for name, (v, ok, _) in candidates.items():
-        if ok and v > value:
-            value, start_used = v, name
-
-    def headline_roofline():
-        if start_used == "tuned":
-            return roofline_of(res["tuned"], res["tune_history"]["history"], "tuned")
-        if start_used == "linear":
-            return roofline_of(res["linear"], res["tune_linear"]["history"], "linear", warm=True)
-        # the learned conv-net path runs two kernels; its solve is reported by the plain one
-        return roofline_of(plain, HISTORY, "plain")
+            "status": status_of(L)}
+    # headline: the fastest FP32 path whose per-tile costs match the plain
+    # run's (m = 5, spectral H0) within 1e-5
+    start_used = "plain"
+    for name, c in candidates.items():
+        if c[1] and c[0] > candidates[start_used][0]:
+            start_used = name
+    value, _, wall_used, hist_used, path_key, warm_used = candidates[start_used]
+    run_used = plain if start_used == "plain" else res[start_used]
     line = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -823,22 +1059,26 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(candidates[start_used][2] * 1e3, 3),
+        "ms_per_step": round(wall_used * 1e3, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (WLC filament scenes, sCMOS camera maps; seeds 1-%d)" % len(frames_np),
-        "config": {"workload": "plain NCS batched: %d frames of %dx%d, alpha %.1f, m=%d "
-                               "reference run, <=%d iterations, eps 1e-5, reference tiling; "
-                               "headline path: %s"
-                               % (len(frames_np), H, W, ALPHA, HISTORY, MAX_IT, start_used),
+        "config": {"workload": "plain NCS batched: %d frames of %dx%d, alpha %.1f, reference "
+                               "tiling; timed path %s: history %d, FP32 MFMA basis changes, "
+                               "run to the stop rule |g| <= 1e-5 max(1, |u|) with at most %d "
+                               "iterations; per-tile costs matched to the m = %d plain run"
+                               % (len(frames_np), H, W, ALPHA, start_used, hist_used, MAX_IT,
+                                  HISTORY),
                    "frames": len(frames_np), "height": H, "width": W, "alpha": ALPHA,
-                   "history": HISTORY, "max_iterations": MAX_IT, "epsilon": 1e-5,
-                   "tiles": int(total), "path": start_used},
+                   "history": hist_used, "reference_history": HISTORY,
+                   "max_iterations": MAX_IT, "epsilon": 1e-5,
+                   "mode": "converged (stop rule), at most %d iterations" % MAX_IT,
+                   "precision": "fp32", "tiles": int(total), "path": start_used},
         "plain": {"value": round(mpix, 2), "unit": "Mpix/s",
                   "ms_per_step": round(plain["wall_s"] * 1e3, 3),
-                  "kernel_ms": round(plain["kernel_ms"], 3),
+                  "kernel_ms": round(plain["kernel_ms"], 3), "history": HISTORY,
                   "initial_hessian": "spectral (precond=1)"},
         "plain_scalar_h0": _scalar_summary(res, plain, pix),
         "tuned": tuned_info,
@@ -847,27 +1087,46 @@ def main():
         "autotune_linear": res.get("tune_linear"),
         "linear_start": linear_info,
         "learned_warm": learned_info,
-        "roofline": headline_roofline(),
-        "tiles_per_s": round(total / candidates[start_used][2], 1),
-        "tiles_per_s_nominal_16x16": round(len(frames_np) * (H // 16) * (W // 16)
-                                           / candidates[start_used][2], 1),
-        "iterations": {"mean": round(float(it.mean()), 3), "p50": float(numpy.median(it)),
-                       "max": int(it.max())},
-        "status": {str(k): int(v) for k, v in zip(*numpy.unique(plain["status"], return_counts=True))},
+        "f16_split": f16_info,
+        "roofline": roofline_of(run_used["kernel_ms"], run_used["iters"], hist_used, path_key,
+                                "fp32", pix, warm=warm_used),
+        "tiles_per_s": round(total / wall_used, 1),
+        "tiles_per_s_nominal_16x16": round(len(frames_np) * (H // 16) * (W // 16) / wall_used, 1),
+        "iterations": {"mean": round(float(run_used["iters"].mean()), 3),
+                       "p50": float(numpy.median(run_used["iters"])),
+                       "max": int(run_used["iters"].max())},
+        "status": status_of(run_used),
         "parity": parity,
         "device": devinfo,
+        "profile_source_hash": solver_source_hash(),
     }
     if "fixed200" in res:
         f = res["fixed200"]
         line["fixed_iterations_mode"] = {
             "value": round(pix / f["wall_s"] / 1e6, 2), "unit": "Mpix/s",
+            "note": "epsilon 0, history %d: every tile runs to 200 iterations or until FP32 "
+                    "can make no further progress" % HISTORY,
             "ms_per_step": round(f["wall_s"] * 1e3, 3), "kernel_ms": round(f["kernel_ms"], 3),
             "iterations_mean": round(float(f["iters"].mean()), 2),
-            "status": {str(k): int(v) for k, v in zip(*numpy.unique(f["status"], return_counts=True))}}
+            "cost_vs_plain_max_excess": float(numpy.max(
+                (f["cost"].astype(numpy.float64) - plain["cost"]) / numpy.abs(plain["cost"]))),
+            "status": status_of(f)}
+    if uneven_frames is not None:
+        u = run_uneven(args, uneven_frames)
+        line["uneven_batch"] = {k: u[k] for k in ("metric", "value", "unit", "ms_per_step",
+                                                  "config", "plain", "linear_start",
+                                                  "pipeline_host_to_host", "iterations",
+                                                  "status", "parity", "roofline",
+                                                  "tiles_per_s", "autotune")}
+    if not args.batch_only:
+        line["stream_series"] = run_stream(args, weights, lin_coef)
+        for k in ("n_gpus", "higher_is_better", "scaling", "vs_baseline", "steps"):
+            line["stream_series"].pop(k, None)
     if not args.no_cpu:
         cb = cpu_baseline(frames_np, args.cpu_budget)
         line["cpu_baseline"] = cb
         line["gpu_over_cpu"] = round(value / cb["value"], 1)
+        line["gpu_over_one_core"] = round(value / cb["one_core"]["value"], 1)
     print(json.dumps(line), flush=True)
 
 

@@ -157,7 +157,11 @@ int ncs_hip_reduce_frames(ncs_ctx *ctx, const ncs_frame *frames, int n_frames,
 /* Same with device pointers (frames[] itself in host memory), enqueued on
  * `stream` (a hipStream_t; 0 is the default NULL stream, as torch's default
  * stream) without waiting for completion.  u0_tiles: device [tiles][256] for NCS_START_TILES, else
- * NULL.  iterations/status/cost: device arrays or NULL. */
+ * NULL.  iterations/status/cost: device arrays or NULL.  The solves of one
+ * context share its frame table and work queue: one enqueued on another
+ * stream than the context's previous solve waits on the device for that
+ * solve to complete, so calls on several streams are safe but run one after
+ * the other (use one context per stream for concurrent solves). */
 int ncs_hip_reduce_frames_device(ncs_ctx *ctx, const ncs_frame *frames,
                                  int n_frames, const ncs_params *params,
                                  const float *u0_tiles, int *iterations,

@@ -128,7 +128,30 @@ struct ncs_ctx {
   // conv net (learned warm start): FP32 parameters and BF16 packs
   DevBuf net_params, net_packs, net_tmp_packs, u0_buf;
   bool has_net = false;
+  // The frame table, work-queue counters and start buffer above are shared
+  // by every solve of the context: a solve enqueued on another stream than
+  // the previous one first waits on the device for that one's completion
+  // event.
+  hipEvent_t solve_done = nullptr;
+  hipStream_t solve_stream = nullptr;
+  bool solve_pending = false;
 };
+
+// Orders a solve on stream s after the context's previous one.
+static hipError_t order_after_previous(ncs_ctx *ctx, hipStream_t s) {
+  if (ctx->solve_pending && ctx->solve_stream != s)
+    return hipStreamWaitEvent(s, ctx->solve_done, 0);
+  return hipSuccess;
+}
+
+static hipError_t mark_solve(ncs_ctx *ctx, hipStream_t s) {
+  hipError_t e = hipEventRecord(ctx->solve_done, s);
+  if (e == hipSuccess) {
+    ctx->solve_stream = s;
+    ctx->solve_pending = true;
+  }
+  return e;
+}
 
 
 static hipError_t set_device(ncs_ctx *ctx) { return hipSetDevice(ctx->device); }
@@ -170,6 +193,7 @@ extern "C" int ncs_hip_create(int device, ncs_ctx **out) {
   std::snprintf(c->arch, sizeof(c->arch), "%s", prop.gcnArchName);
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   HIPCHK(hipEventCreateWithFlags(&c->frames_event, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&c->solve_done, hipEventDisableTiming));
   HIPCHK(c->queue.ensure(kQueueBytes));
   *out = c.release();
   return NCS_OK;
@@ -184,6 +208,7 @@ extern "C" void ncs_hip_destroy(ncs_ctx *ctx) {
                     &ctx->net_params, &ctx->net_packs, &ctx->net_tmp_packs, &ctx->u0_buf};
   for (DevBuf *b : bufs) b->release();
   if (ctx->frames_event) (void)hipEventDestroy(ctx->frames_event);
+  if (ctx->solve_done) (void)hipEventDestroy(ctx->solve_done);
   if (ctx->frames_staging) (void)hipHostFree(ctx->frames_staging);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -380,10 +405,9 @@ static int check_params(const ncs_params *p) {
 }
 
 // Builds kernel frame descriptors (device pointers) and the tile total.
-static int build_frames(const ncs_ctx *ctx, const ncs_frame *frames, int n,
-                        std::vector<FrameDev> &fd, long long &total) {
+static int describe_frames(const ncs_ctx *ctx, const ncs_frame *frames, int n, FrameDev *fd,
+                           long long &total) {
   const int win = ctx->win;
-  fd.resize(n);
   total = 0;
   for (int i = 0; i < n; i++) {
     const ncs_frame &f = frames[i];
@@ -415,6 +439,12 @@ static int build_frames(const ncs_ctx *ctx, const ncs_frame *frames, int n,
   }
   if (total > 0xF0000000LL) return fail(NCS_E_ARG, "too many tiles in one batch");
   return NCS_OK;
+}
+
+static int build_frames(const ncs_ctx *ctx, const ncs_frame *frames, int n,
+                        std::vector<FrameDev> &fd, long long &total) {
+  fd.resize(n);
+  return describe_frames(ctx, frames, n, fd.data(), total);
 }
 
 // Frame descriptors go to the device through a pinned staging copy; a batch
@@ -556,6 +586,7 @@ extern "C" int ncs_hip_reduce_frames_device(ncs_ctx *ctx, const ncs_frame *frame
   if (rc) return rc;
   HIPCHK(set_device(ctx));
   hipStream_t s = (hipStream_t)stream;  // 0 = the default (NULL) stream
+  HIPCHK(order_after_previous(ctx, s));
   rc = upload_frames(ctx, fd, s);
   if (rc) return rc;
   const float *u0 = params->start == NCS_START_TILES ? u0_tiles : nullptr;
@@ -566,9 +597,12 @@ extern "C" int ncs_hip_reduce_frames_device(ncs_ctx *ctx, const ncs_frame *frame
                     ctx->net_packs.p, ctx->net_params.as<float>(), grid, s));
     u0 = ctx->u0_buf.as<float>();
   }
-  return enqueue_solve(ctx, ctx->frames.as<FrameDev>(), n_frames, any_pad(ctx, frames, n_frames),
-                       total, params, u0, nullptr,
-                       iterations, status, cost, ctx->queue.as<unsigned int>(), s);
+  rc = enqueue_solve(ctx, ctx->frames.as<FrameDev>(), n_frames, any_pad(ctx, frames, n_frames),
+                     total, params, u0, nullptr,
+                     iterations, status, cost, ctx->queue.as<unsigned int>(), s);
+  if (rc) return rc;
+  HIPCHK(mark_solve(ctx, s));
+  return NCS_OK;
 }
 
 extern "C" int ncs_hip_solve_tiles_device(ncs_ctx *ctx, const float *data_in,
@@ -622,6 +656,7 @@ extern "C" int ncs_hip_reduce_frames(ncs_ctx *ctx, const ncs_frame *frames, int 
   if (rc) return rc;
   HIPCHK(set_device(ctx));
   hipStream_t s = ctx->stream;
+  HIPCHK(order_after_previous(ctx, s));
   // device layout: each frame's data / out packed [H][W]; gamma deduplicated
   size_t pix = 0, gpix = 0;
   std::vector<size_t> off(n_frames), goff(n_frames);
@@ -679,6 +714,7 @@ extern "C" int ncs_hip_reduce_frames(ncs_ctx *ctx, const ncs_frame *frames, int 
     HIPCHK(hipMemcpyAsync(iterations, ctx->iters.p, total * 4, hipMemcpyDeviceToHost, s));
   if (status) HIPCHK(hipMemcpyAsync(status, ctx->status.p, total * 4, hipMemcpyDeviceToHost, s));
   if (cost) HIPCHK(hipMemcpyAsync(cost, ctx->cost.p, total * 4, hipMemcpyDeviceToHost, s));
+  HIPCHK(mark_solve(ctx, s));
   HIPCHK(hipStreamSynchronize(s));
   return NCS_OK;
 }

@@ -413,3 +413,42 @@ def test_pipeline_learned_start_matches_eager_learned(otf):
     rel = numpy.abs(co_p.astype(numpy.float64) - plain[3]) / numpy.abs(plain[3])
     _say("pipeline learned vs plain: max rel cost", rel.max(), "iters", it_p.mean(), plain[1].mean())
     assert rel.max() <= 1e-5 and numpy.all(st_p == 0)
+
+
+def test_one_context_solves_on_two_streams_in_order(otf):
+    """Device-pointer solves of one context enqueued without host waits on
+    two streams, each batch of a different size (so the shared frame table
+    is rewritten between them): every result equals the same solve made
+    alone."""
+    import torch
+    ctx = otf
+    p = _lib.default_params()
+    dev = torch.device("cuda:0")
+    streams = [torch.cuda.current_stream(), torch.cuda.Stream()]
+    jobs = []
+    for k, shape in enumerate([(96, 130), (150, 64), (64, 64), (120, 200), (80, 90)]):
+        f = sim.make_frame(70 + k, shape)
+        d = torch.from_numpy(f["data"]).to(dev)
+        g = torch.from_numpy(f["gamma"]).to(dev)
+        n = _lib.tile_count(*shape)
+        jobs.append(dict(f=f, d=d, g=g, out=torch.zeros_like(d),
+                         it=torch.zeros(n, dtype=torch.int32, device=dev),
+                         st=torch.zeros(n, dtype=torch.int32, device=dev),
+                         co=torch.zeros(n, dtype=torch.float32, device=dev), shape=shape))
+    torch.cuda.synchronize()
+
+    def enqueue(j, stream):
+        ctx.reduce_frames_device([dict(data=j["d"].data_ptr(), gamma=j["g"].data_ptr(),
+                                       out=j["out"].data_ptr(), height=j["shape"][0],
+                                       width=j["shape"][1], alpha=0.1)], p, None,
+                                 j["it"].data_ptr(), j["st"].data_ptr(), j["co"].data_ptr(),
+                                 stream.cuda_stream)
+
+    for k, j in enumerate(jobs):
+        enqueue(j, streams[k % 2])
+    torch.cuda.synchronize()
+    for k, j in enumerate(jobs):
+        ref = _eager(ctx, [j["f"]["data"]], [j["f"]["gamma"]], [0.1], p)
+        assert numpy.array_equal(j["out"].cpu().numpy(), ref[0][0]), k
+        assert numpy.array_equal(j["co"].cpu().numpy(), ref[3]), k
+        assert numpy.array_equal(j["it"].cpu().numpy(), ref[1]), k
