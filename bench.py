@@ -82,7 +82,7 @@ def _make_frames(n, pool_size):
     return [_frame(s) for s in range(1, n + 1)]
 
 
-SOLVER_SOURCES = ("ncs_amd/csrc/ncs_solver.hip", "ncs_amd/csrc/ncs_common.h", "include/ncs_hip.h")
+SOLVER_SOURCES = ("ncs_amd/csrc/ncs_solver.hip", "ncs_amd/csrc/ncs_common.h")
 
 
 def solver_source_hash():
@@ -148,13 +148,17 @@ def issue_cycles(sq, f16):
     """SIMD issue cycles per launch from the profiled instruction counts.
 
     A wave64 VALU instruction occupies its SIMD's vector datapath for 4
-    cycles (16 lanes per cycle), a transcendental one 16 (quarter rate).  An
-    FP32 MFMA 16x16x4 (2,048 flop) runs on the same datapath for 32 cycles
-    (64 flop per cycle per SIMD: the 157.3 TFLOP/s FP32 matrix peak), so in
-    the FP32 build the two add.  The FP16 build's MFMA 16x16x32 run on the
-    matrix cores beside the VALU; there only the VALU is counted."""
-    valu, mfma, trans = sq["valu"], sq["mfma"], sq.get("trans", 0.0)
-    cycles = 4.0 * (valu - mfma) + 12.0 * trans
+    cycles (16 lanes per cycle); transcendental instructions count as any
+    other (on gfx950 they execute beside the main VALU pipe: charging them
+    at quarter rate would put the measured launches above the issue
+    peak).  An FP32 MFMA 16x16x4 (2,048 flop) runs on the same datapath for
+    32 cycles (64 flop per cycle per SIMD: the 157.3 TFLOP/s FP32 matrix
+    peak; a probe times VALU and FP32 MFMA work together at the sum of their
+    separate times), so in the FP32 build the two add.  The FP16 build's
+    MFMA 16x16x32 run on the matrix cores beside the VALU; there only the
+    VALU is counted."""
+    valu, mfma = sq["valu"], sq["mfma"]
+    cycles = 4.0 * (valu - mfma)
     if not f16:
         cycles += 32.0 * mfma
     return cycles
@@ -185,7 +189,7 @@ def roofline_of(kms, iters, m, path, precision, pix, warm=False, pad=False):
     out = {"bound": "valu", "achieved": None, "peak": round(peak, 1), "unit": "G issue-cycles/s",
            "frac": None, "traffic": int(tr["bytes"]) if tr else None,
            "kernel": kernel, "kernel_ms": round(kms, 4), "precision": precision,
-           "model": "issue cycles = 4 x VALU (non-MFMA) + 12 x transcendental + "
+           "model": "issue cycles = 4 x VALU (non-MFMA) + "
                     + ("0 x MFMA (FP16, matrix cores)" if f16 else "32 x MFMA 16x16x4 f32")
                     + " per launch (SQ counters, profiles/), over SIMDs x 2.4 GHz x launch time",
            "algorithmic_tflops": round(flops / t / 1e12, 3),
@@ -408,6 +412,28 @@ def parity_sample(frames_np, res, ntile_frame, n_sample_frames=1):
     return dict(frames=n_sample_frames, tiles=int(rel.size), max_rel_cost=float(rel.max()),
                 p99_rel_cost=float(numpy.percentile(rel, 99)),
                 max_abs_pixel=dmax, within_1e5=bool(rel.max() <= 1e-5)), el
+
+
+def stalling_oracle_parity(cost, oinfo):
+    """Per-tile costs against the FP64 oracle, whose liblbfgs restatement
+    stops early with a line-search error on some tiles of raw camera frames
+    (clamped pixels with small gamma put the first step of its scalar
+    initial Hessian into the 1e-6 guard): within 1e-5 relative where the
+    oracle converged, no higher where it stopped early."""
+    cost = numpy.asarray(cost, dtype=numpy.float64)
+    conv = numpy.isin(oinfo["status"], (0, 2))
+    rel = (cost - oinfo["cost"]) / numpy.abs(oinfo["cost"])
+    out = {"tiles": int(rel.size), "oracle_converged": int(conv.sum()),
+           "max_rel_cost": float(numpy.abs(rel[conv]).max()),
+           "p99_rel_cost": float(numpy.percentile(numpy.abs(rel[conv]), 99)),
+           "oracle_stopped_early": int((~conv).sum())}
+    ok = out["max_rel_cost"] <= 1e-5
+    if (~conv).any():
+        out["gpu_rel_to_early_stops"] = {"max": float(rel[~conv].max()),
+                                         "min": float(rel[~conv].min())}
+        ok = ok and out["gpu_rel_to_early_stops"]["max"] <= 1e-5
+    out["within_1e5"] = bool(ok)
+    return out
 
 
 def host_threads():
@@ -769,11 +795,7 @@ def run_stream(args, weights, lin_coef=None):
                                    param=oracle.default_param(m=HISTORY, max_iterations=MAX_IT),
                                    nthreads=host_threads()[0])
     for label in results:
-        rel = (numpy.abs(results[label]["costs"][0].astype(numpy.float64) - oinfo["cost"])
-               / numpy.abs(oinfo["cost"]))
-        par[label]["oracle_frame0"] = {"tiles": int(rel.size), "max_rel_cost": float(rel.max()),
-                                       "p99_rel_cost": float(numpy.percentile(rel, 99)),
-                                       "within_1e5": bool(rel.max() <= 1e-5)}
+        par[label]["oracle_frame0"] = stalling_oracle_parity(results[label]["costs"][0], oinfo)
     best = "plain"
     for label in results:
         if (par[label]["max_rel_cost"] <= 1e-5 and par[label]["oracle_frame0"]["within_1e5"]

@@ -188,10 +188,11 @@ def cReduceNoise(image, gamma, otf_mask, alpha, strict=True):
     if im.ndim != 2 or min(im.shape) < r - 2 or numpy.shape(gamma) != im.shape:
         raise NCSCException("image and gamma must be 2-D of one shape, at least %d pixels "
                             "on each side" % (r - 2))
+    # the converted arrays stay bound until the call returns
+    gm, mk = _f64(gamma), _f64(otf_mask)
     ncs_image = numpy.zeros(im.shape, dtype=numpy.float64)
-    _lib.load().ncsReduceNoise(ncs_image.ctypes.data, im.ctypes.data, _f64(gamma).ctypes.data,
-                               _f64(otf_mask).ctypes.data, float(alpha), im.shape[0],
-                               im.shape[1], r)
+    _lib.load().ncsReduceNoise(ncs_image.ctypes.data, im.ctypes.data, gm.ctypes.data,
+                               mk.ctypes.data, float(alpha), im.shape[0], im.shape[1], r)
     return ncs_image
 
 

@@ -15,7 +15,11 @@ size-independent properties:
     within 1e-5 of the plain start's, fewer iterations, fallback rate;
   * time series: raw uint16 2304^2 frames through two captured pipelines,
     plain and linear starts, bit-exact against the eager batch (plain) and
-    against the oracle on a whole frame.
+    against the oracle on a whole frame.  On these frames the oracle's
+    liblbfgs restatement stops early (line-search rounding error) on about
+    0.5 % of the tiles; there the GPU must reach a lower cost at a point the
+    FP64 stop rule accepts, and the GPU run with liblbfgs's own semantics
+    must stop where the oracle stops.
 """
 import os
 
@@ -294,8 +298,46 @@ def test_stream_of_raw_frames_through_two_pipelines(gpu_ctx):
                                              MASK_FFT.astype(numpy.float64), ALPHA,
                                              nthreads=NTHREADS)
             plain0 = res[0][2].astype(numpy.float64)
-        rel = numpy.abs(res[0][2].astype(numpy.float64) - rinfo["cost"]) / numpy.abs(rinfo["cost"])
-        _say("  frame 0 against the oracle (%d tiles): cost max rel %.2e" % (len(rel), rel.max()))
-        assert rel.max() <= 1e-5
+        _check_against_stalling_oracle(ctx, res[0][2], rinfo, ph[0], gamma, p)
         if label == "linear":
             assert numpy.max(numpy.abs(res[0][2] - plain0) / numpy.abs(plain0)) <= 1e-5
+    # the same frame with liblbfgs's semantics (scalar initial Hessian,
+    # history 6, no cap) reproduces the oracle on every tile, its early stops
+    # included
+    o = numpy.zeros(shape, numpy.float32)
+    it_l, st_l, c_l = ctx.reduce_frames([ph[0]], [gamma], [o], [ALPHA], _lib.GEOM_REFERENCE,
+                                        _lib.liblbfgs_params(max_iterations=100000))
+    rel = numpy.abs(c_l.astype(numpy.float64) - rinfo["cost"]) / numpy.abs(rinfo["cost"])
+    _say("  liblbfgs semantics: early stops GPU %d oracle %d, cost max rel %.2e"
+         % (int(numpy.sum(st_l != 0)), int(numpy.sum(rinfo["status"] != 0)), rel.max()))
+    assert rel.max() <= 1e-5
+
+
+def _check_against_stalling_oracle(ctx, cost, rinfo, data, gamma, params):
+    """Per-tile parity with the FP64 oracle, whose liblbfgs restatement can
+    stop early with a line-search error (LBFGSERR_ROUNDING_ERROR) where
+    clamped pixels (data 1e-6) with small gamma put the first step of the
+    scalar initial Hessian into the 1e-6 guard: where the oracle converged,
+    costs within 1e-5 relative; where it stopped early, the GPU's cost no
+    higher, and the GPU's point stationary by the FP64 stop rule."""
+    from ncs_amd import warmstart
+    cost = numpy.asarray(cost, dtype=numpy.float64)
+    conv = numpy.isin(rinfo["status"], (oracle.LBFGS_SUCCESS, oracle.LBFGS_ALREADY_MINIMIZED))
+    rel = (cost - rinfo["cost"]) / numpy.abs(rinfo["cost"])
+    _say("  oracle converged on %d tiles: cost max |rel| %.2e; stopped early on %d: GPU "
+         "lower by %.2e .. %.2e" % (int(conv.sum()), numpy.abs(rel[conv]).max(),
+                                    int((~conv).sum()),
+                                    -rel[~conv].max() if (~conv).any() else 0.0,
+                                    -rel[~conv].min() if (~conv).any() else 0.0))
+    assert numpy.abs(rel[conv]).max() <= 1e-5
+    if (~conv).any():
+        assert rel[~conv].max() <= 1e-5
+        idx = numpy.nonzero(~conv)[0]
+        dw = warmstart.reference_windows(data)[idx]
+        gw = warmstart.reference_windows(gamma)[idx]
+        # the same tiles through the tile entry point (its padded-block build
+        # rounds differently in the last bit)
+        u, _, st, c = ctx.solve_tiles(dw, gw, ALPHA, params)
+        assert numpy.max(numpy.abs(c - cost[idx]) / numpy.abs(cost[idx])) <= 1e-6
+        _, gn, un = oracle.eval_cost_f32(u, dw, gw, MASK_FFT, ALPHA)
+        assert numpy.all(st == 0) and numpy.all(gn <= 2e-5 * numpy.maximum(1.0, un))

@@ -285,10 +285,19 @@ def test_window_side_r_tile_costs_match_oracle(r, kind, _restore_16):
         # reference either (pyOpenCLNCS/ncs.py:159-162); the cost bar decides
         assert numpy.all((info["status"] == 0) | (info["status"] == -5))
     assert rel.max() <= 1e-5
-    with pytest.raises(_lib.NCSHipError):  # 16 x 16 tiles only
-        _lib.context(0).solve_tiles(numpy.ones((2, 16, 16), numpy.float32),
-                                    numpy.ones((2, 16, 16), numpy.float32), 0.1,
-                                    _lib.default_params())
+    # the tile entry point takes tiles of the mask's side
+    ctx = _lib.context(0)
+    with pytest.raises(ValueError):
+        ctx.solve_tiles(numpy.ones((2, 16, 16), numpy.float32),
+                        numpy.ones((2, 16, 16), numpy.float32), 0.1, _lib.default_params())
+    dt = numpy.stack([data[k:k + r, 2 * k:2 * k + r] for k in range(8)]).astype(numpy.float32)
+    gt = numpy.stack([gamma[k:k + r, 2 * k:2 * k + r] for k in range(8)]).astype(numpy.float32)
+    ut, itt, stt, ct = ctx.solve_tiles(dt, gt, alpha, _lib.default_params())
+    tref = oracle.solve_tiles(dt, gt, masks.to_fft_order(mask).astype(numpy.float32), alpha,
+                              f32=True)
+    trel = numpy.abs(ct - tref["cost"]) / numpy.abs(tref["cost"])
+    _say("r x r tiles: cost max rel", float(trel.max()))
+    assert ut.shape == dt.shape and trel.max() <= 1e-5
     with pytest.raises(_lib.NCSHipError):  # the plain start only
         h.params = _lib.default_params(start=_lib.START_LINEAR)
         h.reduceNoise([data], alpha)
