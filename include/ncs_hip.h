@@ -167,6 +167,19 @@ int ncs_hip_reduce_frames_device(ncs_ctx *ctx, const ncs_frame *frames,
                                  const float *u0_tiles, int *iterations,
                                  int *status, float *cost, void *stream);
 
+/* ncs_hip_reduce_frames_device without allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * work-queue counters and the frame table live in a fixed device array of
+ * the library (19,456 bytes per GPU, held until the process ends), written
+ * by kernels; at most 256 frames per call; the learned (conv-net) start,
+ * which needs a start buffer, is refused.  Every call on one GPU shares that
+ * array: calls that overlap on one GPU race, whatever their context. */
+int ncs_hip_reduce_frames_device_noalloc(ncs_ctx *ctx, const ncs_frame *frames,
+                                         int n_frames, const ncs_params *params,
+                                         const float *u0_tiles, int *iterations,
+                                         int *status, float *cost, void *stream);
+
 /* Independent 16x16 tiles [num_tiles][16][16] (r x r tiles [num_tiles][r][r]
  * after an r x r mask), host memory; synchronous.  Replaces the kernel ABI
  * ncsReduceNoise(data_in, g_gamma, otf_mask, data_out, g_iterations,

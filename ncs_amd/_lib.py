@@ -120,6 +120,7 @@ SIGNATURES = {
     "ncs_hip_tile_count_window": (_ll, [_i, _i, _i, _i]),
     "ncs_hip_reduce_frames": (_i, [_vp, _F, _i, _P, _vp, _vp, _vp]),
     "ncs_hip_reduce_frames_device": (_i, [_vp, _F, _i, _P, _vp, _vp, _vp, _vp, _vp]),
+    "ncs_hip_reduce_frames_device_noalloc": (_i, [_vp, _F, _i, _P, _vp, _vp, _vp, _vp, _vp]),
     "ncs_hip_solve_tiles": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _f, _P]),
     "ncs_hip_solve_tiles_device": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _f, _P,
                                         _vp]),

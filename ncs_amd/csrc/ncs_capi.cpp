@@ -135,6 +135,8 @@ struct ncs_ctx {
   hipEvent_t solve_done = nullptr;
   hipStream_t solve_stream = nullptr;
   bool solve_pending = false;
+  // device address of the allocation-free entry point's scratch on this GPU
+  void *scratch = nullptr;
 };
 
 // Orders a solve on stream s after the context's previous one.
@@ -155,6 +157,56 @@ static hipError_t mark_solve(ncs_ctx *ctx, hipStream_t s) {
 
 
 static hipError_t set_device(ncs_ctx *ctx) { return hipSetDevice(ctx->device); }
+
+// ---------------------------------------------------------------------------
+// device scratch of the allocation-free entry point
+// ---------------------------------------------------------------------------
+
+// ncs_hip_reduce_frames_device_noalloc keeps its work-queue counters and its
+// frame table in this fixed array instead of context buffers: kQueueBytes
+// of counters, then up to kScratchFrames frame descriptors (19,456 bytes).
+// Each GPU holds its own copy until the process ends; every call on a GPU
+// shares it.
+constexpr int kScratchFrames = 256;
+constexpr size_t kScratchFrameOffset = 1024;
+constexpr size_t kScratchBytes = kScratchFrameOffset + kScratchFrames * sizeof(FrameDev);
+static_assert(kQueueBytes <= kScratchFrameOffset, "queue counters overflow their slot");
+static_assert(kScratchFrameOffset % alignof(FrameDev) == 0, "misaligned frame table");
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch[kMaxDevices] = {};
+
+// Device address of `scratch` on `device`, which must be current; looked up
+// once per device.  Host code uses this address, never the array's name.
+static int scratch_on_device(int device, void **out) {
+  if (device < 0 || device >= kMaxDevices) return fail(NCS_E_ARG, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(NCS_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch[device] = p;
+  }
+  *out = g_scratch[device];
+  return NCS_OK;
+}
+
+// Frame descriptors passed by value as a kernel argument and written to the
+// scratch frame table: no host staging buffer and no copy from host memory.
+constexpr int kFramesPerWrite = 24;
+struct FrameChunk {
+  int first, n;
+  FrameDev f[kFramesPerWrite];
+};
+static_assert(sizeof(FrameChunk) <= 4096, "kernel argument too large");
+
+__global__ void write_frames_kernel(FrameChunk c, FrameDev *dst) {
+  const int i = threadIdx.x;
+  if (i < c.n) dst[c.first + i] = c.f[i];
+}
 
 extern "C" void ncs_hip_default_params(ncs_params *p) {
   p->max_iterations = 200;
@@ -195,6 +247,8 @@ extern "C" int ncs_hip_create(int device, ncs_ctx **out) {
   HIPCHK(hipEventCreateWithFlags(&c->frames_event, hipEventDisableTiming));
   HIPCHK(hipEventCreateWithFlags(&c->solve_done, hipEventDisableTiming));
   HIPCHK(c->queue.ensure(kQueueBytes));
+  const int rc = scratch_on_device(device, &c->scratch);
+  if (rc) return rc;
   *out = c.release();
   return NCS_OK;
 }
@@ -603,6 +657,54 @@ extern "C" int ncs_hip_reduce_frames_device(ncs_ctx *ctx, const ncs_frame *frame
   if (rc) return rc;
   HIPCHK(mark_solve(ctx, s));
   return NCS_OK;
+}
+
+// ncs_hip_reduce_frames_device without allocation: the frame table and the
+// work-queue counters live in the device scratch array, the frame table is
+// written by kernels from their arguments, and nothing is allocated.  Calls
+// that overlap on one GPU race on the scratch, whatever their context.
+extern "C" int ncs_hip_reduce_frames_device_noalloc(ncs_ctx *ctx, const ncs_frame *frames,
+                                                    int n_frames, const ncs_params *params,
+                                                    const float *u0_tiles, int *iterations,
+                                                    int *status, float *cost, void *stream) {
+  if (!ctx || !frames || n_frames <= 0) return fail(NCS_E_ARG, "bad arguments");
+  if (n_frames > kScratchFrames)
+    return fail(NCS_E_ARG, "the allocation-free entry point takes at most " +
+                               std::to_string(kScratchFrames) + " frames per call (got " +
+                               std::to_string(n_frames) + ")");
+  int rc = check_params(params);
+  if (rc) return rc;
+  rc = check_window(ctx, params);
+  if (rc) return rc;
+  if (params->start == NCS_START_LEARNED)
+    return fail(NCS_E_ARG, "the learned start needs the context's start buffer: use "
+                           "ncs_hip_reduce_frames_device");
+  if (params->start == NCS_START_TILES && !u0_tiles)
+    return fail(NCS_E_ARG, "NCS_START_TILES needs u0_tiles");
+  if (params->start == NCS_START_LINEAR && !ctx->has_linear)
+    return fail(NCS_E_NOWEIGHTS, "linear start requested but no coefficients are set");
+  if (!ctx->scratch) return fail(NCS_E_HIP, "no device scratch on this context's GPU");
+  FrameDev fd[kScratchFrames];
+  long long total = 0;
+  rc = describe_frames(ctx, frames, n_frames, fd, total);
+  if (rc) return rc;
+  HIPCHK(set_device(ctx));
+  hipStream_t s = (hipStream_t)stream;
+  unsigned int *dq = static_cast<unsigned int *>(ctx->scratch);
+  FrameDev *dfr =
+      reinterpret_cast<FrameDev *>(static_cast<unsigned char *>(ctx->scratch) + kScratchFrameOffset);
+  for (int i0 = 0; i0 < n_frames; i0 += kFramesPerWrite) {
+    FrameChunk c;
+    c.first = i0;
+    c.n = std::min(kFramesPerWrite, n_frames - i0);
+    for (int k = 0; k < c.n; k++) c.f[k] = fd[i0 + k];
+    hipLaunchKernelGGL(write_frames_kernel, dim3(1), dim3(64), 0, s, c, dfr);
+    HIPCHK(hipGetLastError());
+  }
+  // the counters are reset by a kernel before the solve (launch_solve)
+  return enqueue_solve(ctx, dfr, n_frames, any_pad(ctx, frames, n_frames), total, params,
+                       params->start == NCS_START_TILES ? u0_tiles : nullptr, nullptr,
+                       iterations, status, cost, dq, s);
 }
 
 extern "C" int ncs_hip_solve_tiles_device(ncs_ctx *ctx, const float *data_in,
