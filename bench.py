@@ -960,10 +960,13 @@ def main():
     res, ntile_frame, total, devinfo = run_gpu(frames_np, args, weights, lin_coef)
     if args.only:
         r = res[args.only]
-        print(json.dumps({"path": args.only, "kernel_ms": r["kernel_ms"],
-                          "ms_per_step": r["wall_s"] * 1e3, "steps": r["steps"],
-                          "iterations_mean": float(r["iters"].mean()),
-                          "source_hash": solver_source_hash()}), flush=True)
+        line = {"path": args.only, "kernel_ms": r["kernel_ms"],
+                "ms_per_step": r["wall_s"] * 1e3, "steps": r["steps"],
+                "iterations_mean": float(r["iters"].mean()),
+                "source_hash": solver_source_hash()}
+        if "net_ms" in r:
+            line["net_ms"] = r["net_ms"]
+        print(json.dumps(line), flush=True)
         return
     plain = res["plain"]
     pix = len(frames_np) * H * W

@@ -30,6 +30,7 @@ hipError_t net_forward(const float *x, float *y, void *acts, const void *packs, 
                        long long n, int grid, hipStream_t s);
 hipError_t net_backward(const float *x, const void *acts, const float *gout, float *gP,
                         const void *packs, long long n, int grid, hipStream_t s);
+int net_forward_grid(long long n, int cus);
 hipError_t net_warm(const FrameDev *frames, int n_frames, long long total, float *u0,
                     const void *packs, const float *P, int grid, hipStream_t s);
 hipError_t launch_eval(bool general, const float *data, const float *gamma, const float *u,
@@ -646,7 +647,7 @@ extern "C" int ncs_hip_reduce_frames_device(ncs_ctx *ctx, const ncs_frame *frame
   const float *u0 = params->start == NCS_START_TILES ? u0_tiles : nullptr;
   if (params->start == NCS_START_LEARNED) {
     HIPCHK(ctx->u0_buf.ensure((size_t)total * kTilePix * sizeof(float)));
-    int grid = (int)std::min<long long>(total, (long long)ctx->cus * 4);
+    const int grid = net_forward_grid(total, ctx->cus);
     HIPCHK(net_warm(ctx->frames.as<FrameDev>(), n_frames, total, ctx->u0_buf.as<float>(),
                     ctx->net_packs.p, ctx->net_params.as<float>(), grid, s));
     u0 = ctx->u0_buf.as<float>();
@@ -918,7 +919,7 @@ extern "C" int ncs_hip_net_predict_frames_device(ncs_ctx *ctx, const ncs_frame *
   hipStream_t s = (hipStream_t)stream;
   rc = upload_frames(ctx, fd, s);
   if (rc) return rc;
-  int grid = (int)std::max<long long>(1, std::min<long long>(total, (long long)ctx->cus * 4));
+  const int grid = net_forward_grid(total, ctx->cus);
   HIPCHK(net_warm(ctx->frames.as<FrameDev>(), n_frames, total, u0_tiles, ctx->net_packs.p,
                   ctx->net_params.as<float>(), grid, s));
   return NCS_OK;
@@ -933,7 +934,7 @@ extern "C" int ncs_hip_net_forward(ncs_ctx *ctx, const float *params, const floa
   const float *P;
   int rc = packs_for(ctx, params, s, &packs, &P);
   if (rc) return rc;
-  int grid = (int)std::max<long long>(1, std::min<long long>(n, (long long)ctx->cus * 4));
+  const int grid = net_forward_grid(n, ctx->cus);
   HIPCHK(net_forward(x, out, acts, packs, P, n, grid, s));
   return NCS_OK;
 }
@@ -1410,7 +1411,7 @@ static int capture_replay(ncs_pipeline *pl, hipGraph_t *out) {
   }
   const float *u0 = nullptr;
   if (e == hipSuccess && pl->learned) {
-    int grid = (int)std::min<long long>(pl->total, (long long)ctx->cus * 4);
+    const int grid = net_forward_grid(pl->total, ctx->cus);
     e = net_warm(pl->frames.as<FrameDev>(), pl->n, pl->total, pl->u0.as<float>(),
                  ctx->net_packs.p, ctx->net_params.as<float>(), grid, s);
     u0 = pl->u0.as<float>();

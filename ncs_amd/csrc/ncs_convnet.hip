@@ -11,16 +11,18 @@
 // y and a block of 16 output channels,
 //   D[cout][x] += sum_taps  W_tap[cout][cin] * In[cin][y + dy][x + dx],
 // A = weights (pre-packed per tap and block: lane (g, i) holds cout i,
-// cin 8g..8g+7), B = activations (lane (g, x) holds cin 8g..8g+7 of pixel
-// x), D: lane (g, x) holds couts 4g..4g+3 of pixel x.  Activations live in
-// LDS as [pixel][32] BF16, one 1 KB row per read; the dx = -1 / +1 taps are
-// DPP row_shr:1 / row_shl:1 of the same row, whose zero fill at the row
-// end is exactly the zero padding of the tile edge.  A 3-row sliding
-// window keeps every input row read once per layer.
+// K chunk 8g..8g+7), B = activations (lane (g, x) holds K chunk g of pixel
+// x), D: lane (g, x) holds couts 4g..4g+3 of pixel x.  In the forward pass a
+// tile's activations stay in registers (16 rows x 4 VGPRs per lane, see
+// fwd_channel); the dx = -1 / +1 taps are DPP row_shr:1 / row_shl:1 of a
+// row, whose zero fill at the row end is exactly the zero padding of the
+// tile edge.  LDS holds only the input grid and layer 4's partial sums.
 //
 // Backward: the data gradient is the same convolution with flipped,
 // transposed weight packs; the weight gradient is a GEMM over pixels
 // (MFMA with gathered operands), accumulated into FP32 with atomics.
+#include <algorithm>
+
 #include "ncs_common.h"
 
 namespace ncs {
@@ -35,15 +37,26 @@ constexpr int kCh = 32;
 constexpr int OFF_W1 = 0, OFF_B1 = 576, OFF_W2 = 608, OFF_B2 = 9824, OFF_W3 = 9856,
               OFF_B3 = 19072, OFF_W4 = 19104, OFF_B4 = 19392, N_PARAMS = 19393;
 static_assert(N_PARAMS == NCS_NET_PARAMS, "parameter count");
-// A-operand packs (64 lanes x 8 BF16 = 1 KB each)
-constexpr int PK_L1 = 0;   // 2: layer 1, K = (tap, cin) padded to 32
-constexpr int PK_L2 = 2;   // 18: block * 9 + tap
-constexpr int PK_L3 = 20;  // 18
-constexpr int PK_L4 = 38;  // 9: tap (cout 0 in row 0)
-constexpr int PK_B4 = 47;  // 2: layer-4 data gradient, K = tap padded to 32
-constexpr int PK_B3 = 49;  // 18: layer-3 data gradient (flipped, transposed)
-constexpr int PK_B2 = 67;  // 18: layer-2 data gradient
-constexpr int N_PACKS = 85;
+// MFMA operand packs (64 lanes x 8 BF16 = 1 KB each)
+constexpr int PK_L1 = 0;    // 2: layer 1 A, K = (tap, cin) padded to 32
+constexpr int PK_L2 = 2;    // 18: layer 2 A, block * 9 + tap, K order fwd_channel
+constexpr int PK_L3 = 20;   // 18
+constexpr int PK_L4T = 38;  // 3: layer 4 B per dy, column n = dx (n < 3), K order fwd_channel
+constexpr int PK_B4 = 41;   // 2: layer-4 data gradient, K = tap padded to 32
+constexpr int PK_B3 = 43;   // 18: layer-3 data gradient (flipped, transposed)
+constexpr int PK_B2 = 61;   // 18: layer-2 data gradient
+constexpr int N_PACKS = 79;
+
+// Forward activations live in registers.  With the K index k = 8g + e of a
+// 32-channel layer's operand mapped to channel fwd_channel(k) (e < 4: 4g + e,
+// else 16 + 4g + e - 4), the D fragments of a row's two output blocks at lane
+// (g, x) -- couts 4g..4g+3, then 16 + 4g..16 + 4g + 3, of pixel x -- are in
+// that order exactly the 8-channel fragment lane (g, x) supplies to the next
+// layer.  A tile's activations are 16 rows x 4 VGPRs per lane and never cross
+// lanes; the packs of layers 2-4 carry the permutation in their K order.
+__host__ __device__ constexpr int fwd_channel(int k) {
+  return (k & 7) < 4 ? 4 * (k >> 3) + (k & 7) : 12 + 4 * (k >> 3) + (k & 7);
+}
 
 __device__ __forceinline__ float w_at(const float *P, int off, int cin_n, int co, int ci,
                                       int tap) {
@@ -63,13 +76,13 @@ __global__ void net_pack_kernel(const float *__restrict__ P, __bf16 *__restrict_
     if (k < 18) v = w_at(P, OFF_W1, 2, co, k & 1, k >> 1);
   } else if (pack < PK_L3) {
     const int q = pack - PK_L2, co = 16 * (q / 9) + i;
-    v = w_at(P, OFF_W2, kCh, co, k, q % 9);
-  } else if (pack < PK_L4) {
+    v = w_at(P, OFF_W2, kCh, co, fwd_channel(k), q % 9);
+  } else if (pack < PK_L4T) {
     const int q = pack - PK_L3, co = 16 * (q / 9) + i;
-    v = w_at(P, OFF_W3, kCh, co, k, q % 9);
-  } else if (pack < PK_B4) {
-    const int tap = pack - PK_L4;
-    if (i == 0) v = w_at(P, OFF_W4, kCh, 0, k, tap);
+    v = w_at(P, OFF_W3, kCh, co, fwd_channel(k), q % 9);
+  } else if (pack < PK_B4) {  // B operand: lane (g, n) holds K = cin 8g..8g+7 of column n
+    const int dy = pack - PK_L4T;
+    if (i < 3) v = w_at(P, OFF_W4, kCh, 0, fwd_channel(k), dy * 3 + i);
   } else if (pack < PK_B3) {  // d(layer-3 act) from d(out): cout' = L4 cin, k = tap
     const int co = 16 * (pack - PK_B4) + i;
     if (k < 9) v = w_at(P, OFF_W4, kCh, 0, co, 8 - k);
@@ -97,6 +110,23 @@ __device__ __forceinline__ i4 dpp_i4(i4 v) {
   r.w = __builtin_amdgcn_update_dpp(0, v.w, CTRL, 0xF, 0xF, true);
   return r;
 }
+// The same pointer, opaque to the optimiser: loads through it are not hoisted
+// out of the per-tile loop (the operand packs of all four layers would not
+// fit in registers beside the working set).
+template <class T>
+__device__ __forceinline__ const T *opaque(const T *p) {
+  asm volatile("" : "+s"(p));
+  return p;
+}
+
+// Accumulator start of the 16-channel block b: the bias of the output
+// channels lane (g, x) holds (16 b + 4 g + r, r = 0..3).
+__device__ __forceinline__ f4 bias4(const float *bias, int b, int lane) {
+  const int g = lane >> 4;
+  return (f4){bias[16 * b + 4 * g], bias[16 * b + 4 * g + 1], bias[16 * b + 4 * g + 2],
+              bias[16 * b + 4 * g + 3]};
+}
+
 constexpr int kRowShr1 = 0x111;  // lane x <- lane x-1 in its row (0 at x = 0)
 constexpr int kRowShl1 = 0x101;  // lane x <- lane x+1 in its row (0 at x = 15)
 
@@ -163,12 +193,187 @@ __device__ __forceinline__ void conv32(const __bf16 *in, const i4 *__restrict__ 
   }
 }
 
+// ReLU of one row's two block accumulators as a BF16 register row.
+__device__ __forceinline__ i4 relu_row(f4 a0, f4 a1) {
+  bf16x8 h;
+  h[0] = (__bf16)fmaxf(a0.x, 0.0f);
+  h[1] = (__bf16)fmaxf(a0.y, 0.0f);
+  h[2] = (__bf16)fmaxf(a0.z, 0.0f);
+  h[3] = (__bf16)fmaxf(a0.w, 0.0f);
+  h[4] = (__bf16)fmaxf(a1.x, 0.0f);
+  h[5] = (__bf16)fmaxf(a1.y, 0.0f);
+  h[6] = (__bf16)fmaxf(a1.z, 0.0f);
+  h[7] = (__bf16)fmaxf(a1.w, 0.0f);
+  return __builtin_bit_cast(i4, h);
+}
+
+// A register row written as row y of a [256][32] BF16 buffer in the
+// standard channel order (the activations kept for the backward pass).
+__device__ __forceinline__ void store_row_std(__bf16 *dst, int y, int lane, i4 v) {
+  const int x = lane & 15, g = lane >> 4;
+  __bf16 *px = dst + (y * 16 + x) * kCh;
+  *reinterpret_cast<int2 *>(px + 4 * g) = make_int2(v.x, v.y);
+  *reinterpret_cast<int2 *>(px + 16 + 4 * g) = make_int2(v.z, v.w);
+}
+
+// Layer 1 (2 -> 32 channels), K = tap * 2 + cin padded to 32.  The inputs sit
+// in LDS as an 18 x 18 grid of (x0, x1) float pairs with a zero border, so
+// lane (g, x) reads its K = 8g..8g+7 (taps 4g..4g+3) of row y as four float2
+// at fixed offsets; taps past 8 read the zero border column of the row.
+__device__ __forceinline__ int in_pad_offset(int tap, int x) {
+  return tap < 9 ? ((tap / 3) * 18 + x + tap % 3) * 2 : 0;
+}
+
+template <class Keep>
+__device__ __forceinline__ void layer_in2(const float *in_pad, const i4 *__restrict__ packs,
+                                          const float *bias, int lane, i4 (&act)[16],
+                                          Keep keep) {
+  const i4 A0 = packs[lane], A1 = packs[64 + lane];
+  const f4 c0 = bias4(bias, 0, lane), c1 = bias4(bias, 1, lane);
+  const int x = lane & 15, g = lane >> 4;
+  int off[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) off[j] = in_pad_offset(4 * g + j, x);
+#pragma unroll
+  for (int y = 0; y < 16; y++) {
+    bf16x8 bv;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const float2 v = *reinterpret_cast<const float2 *>(in_pad + y * 36 + off[j]);
+      bv[2 * j] = (__bf16)v.x;
+      bv[2 * j + 1] = (__bf16)v.y;
+    }
+    const i4 B = __builtin_bit_cast(i4, bv);
+    act[y] = relu_row(mfma_bf16(A0, B, c0), mfma_bf16(A1, B, c1));
+    keep(y, act[y]);
+  }
+}
+
+// A 32 -> 32 channel layer in place over the register rows, two output rows
+// per step (four independent accumulation chains: two rows x two blocks).
+// Input rows y - 1 .. y + 2 and their dx = -1 / +1 shifts (DPP row_shr:1 /
+// row_shl:1, whose zero fill at the row end is the tile's zero padding) are
+// held as values when rows y and y + 1 are overwritten; rows y + 1 and y + 2
+// carry over to the next step, so each row is shifted once per layer.
+template <class Keep>
+__device__ __forceinline__ void layer32(i4 (&act)[16], const i4 *__restrict__ packs_in,
+                                        const float *bias_in, int lane, Keep keep) {
+  const i4 *packs = opaque(packs_in);
+  const float *bias = opaque(bias_in);
+  i4 A[2][9];
+#pragma unroll
+  for (int b = 0; b < 2; b++)
+#pragma unroll
+    for (int t = 0; t < 9; t++) A[b][t] = packs[(b * 9 + t) * 64 + lane];
+  const f4 c0 = bias4(bias, 0, lane), c1 = bias4(bias, 1, lane);
+  const i4 z = {0, 0, 0, 0};
+  i4 C[4], L[4], R[4];  // rows y - 1 .. y + 2: centre, dx = -1, dx = +1
+  C[0] = L[0] = R[0] = z;
+  C[1] = act[0];
+  L[1] = dpp_i4<kRowShr1>(C[1]);
+  R[1] = dpp_i4<kRowShl1>(C[1]);
+#pragma unroll
+  for (int y = 0; y < 16; y += 2) {
+    C[2] = act[y + 1];
+    L[2] = dpp_i4<kRowShr1>(C[2]);
+    R[2] = dpp_i4<kRowShl1>(C[2]);
+    if (y + 2 < 16) {
+      C[3] = act[y + 2];
+      L[3] = dpp_i4<kRowShr1>(C[3]);
+      R[3] = dpp_i4<kRowShl1>(C[3]);
+    }
+    f4 acc[2][2] = {{c0, c1}, {c0, c1}};
+#pragma unroll
+    for (int q = 0; q < 4; q++) {  // input row y - 1 + q
+      if ((q == 0 && y == 0) || (q == 3 && y + 2 >= 16)) continue;  // zero rows
+#pragma unroll
+      for (int dx = 0; dx < 3; dx++) {
+        const i4 in = dx == 0 ? L[q] : (dx == 1 ? C[q] : R[q]);
+#pragma unroll
+        for (int o = 0; o < 2; o++) {  // output row y + o: dy = q - o
+          const int dy = q - o;
+          if (dy < 0 || dy > 2) continue;
+#pragma unroll
+          for (int b = 0; b < 2; b++) acc[o][b] = mfma_bf16(A[b][dy * 3 + dx], in, acc[o][b]);
+        }
+      }
+    }
+    act[y] = relu_row(acc[0][0], acc[0][1]);
+    keep(y, act[y]);
+    act[y + 1] = relu_row(acc[1][0], acc[1][1]);
+    keep(y + 1, act[y + 1]);
+    C[0] = C[2];
+    L[0] = L[2];
+    R[0] = R[2];
+    C[1] = C[3];
+    L[1] = L[3];
+    R[1] = R[3];
+  }
+}
+
+// Layer 4 (32 -> 1 channel) with the register rows as the A operand: lane
+// (g, x) holds A[x][8g..8g+7] (pixel x, K chunk g), and B_dy[k][n] =
+// W4[fwd_channel(k)][dy][n] for n = dx < 3.  Three chained MFMA per row give
+// D_y[x'][n] = sum_dy sum_ci a3[ci][y + dy - 1][x'] W4[ci][dy][n] (D lane
+// (g, n) holds x' = 4g..4g+3); all 16 rows go to dbuf ([16][16][3] FP32 LDS)
+// and, after one barrier,
+//   out[y][x] = b4 + D_y[x - 1][0] + D_y[x][1] + D_y[x + 1][2]
+// (zero past the tile) for the pixels (4g + r, x) of lane (g, x), r = 0..3.
+__device__ __forceinline__ f4 layer_out1(const i4 (&act)[16], const i4 *__restrict__ packs,
+                                         float *dbuf, float b4, int lane) {
+  const i4 B0 = packs[lane], B1 = packs[64 + lane], B2 = packs[128 + lane];
+  const int n = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int y = 0; y < 16; y++) {
+    f4 d = {0.f, 0.f, 0.f, 0.f};
+    if (y > 0) d = mfma_bf16(act[y - 1], B0, d);
+    d = mfma_bf16(act[y], B1, d);
+    if (y < 15) d = mfma_bf16(act[y + 1], B2, d);
+    if (n < 3)
+#pragma unroll
+      for (int r = 0; r < 4; r++) dbuf[(y * 16 + 4 * g + r) * 3 + n] = d[r];
+  }
+  __syncthreads();
+  f4 out;
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const float *D = dbuf + (4 * g + r) * 48;
+    float v = b4 + D[n * 3 + 1];
+    if (n > 0) v += D[(n - 1) * 3 + 0];
+    if (n < 15) v += D[(n + 1) * 3 + 2];
+    out[r] = v;
+  }
+  return out;
+}
+
+// Writes the zero border of the 18 x 18 input grid (68 cells).
+__device__ __forceinline__ void in_pad_border(float *in_pad, int lane) {
+  for (int c = lane; c < 68; c += 64) {
+    int cell;
+    if (c < 18) cell = c;                          // top row
+    else if (c < 36) cell = 17 * 18 + (c - 18);    // bottom row
+    else if (c < 52) cell = (c - 35) * 18;         // left column, rows 1..16
+    else cell = (c - 51) * 18 + 17;                // right column, rows 1..16
+    *reinterpret_cast<float2 *>(in_pad + 2 * cell) = make_float2(0.0f, 0.0f);
+  }
+}
+
+__device__ __forceinline__ void in_pad_put(float *in_pad, int row, int col, float x0, float x1) {
+  *reinterpret_cast<float2 *>(in_pad + 2 * ((row + 1) * 18 + col + 1)) = make_float2(x0, x1);
+}
+
 // A layer whose input has NC <= 3 channels (K = tap * NC + c padded to 32),
-// input FP32 in LDS as [NC][256]; 2 output blocks.
+// input FP32 in LDS as [NC][256]; 2 output blocks, accumulators starting
+// from `bias` (zero when null).
 template <int NC, class Epi>
 __device__ __forceinline__ void conv_small(const float *in, const i4 *__restrict__ packs,
-                                           int lane, Epi epi) {
+                                           int lane, Epi epi, const float *bias = nullptr) {
   const i4 A0 = packs[lane], A1 = packs[64 + lane];
+  f4 c0 = {0.f, 0.f, 0.f, 0.f}, c1 = c0;
+  if (bias != nullptr) {
+    c0 = bias4(bias, 0, lane);
+    c1 = bias4(bias, 1, lane);
+  }
   const int x = lane & 15, g = lane >> 4;
   for (int y = 0; y < 16; y++) {
     bf16x8 bv;
@@ -184,17 +389,15 @@ __device__ __forceinline__ void conv_small(const float *in, const i4 *__restrict
       bv[e] = (__bf16)v;
     }
     const i4 B = __builtin_bit_cast(i4, bv);
-    f4 z = {0.f, 0.f, 0.f, 0.f};
-    epi(y, 0, mfma_bf16(A0, B, z));
-    epi(y, 1, mfma_bf16(A1, B, z));
+    epi(y, 0, mfma_bf16(A0, B, c0));
+    epi(y, 1, mfma_bf16(A1, B, c1));
   }
 }
 
-// Gathers one tile in the solver's reference tiling and builds the two
-// normalised input channels into in2 ([2][256] FP32 LDS); returns mu/sigma
-// and the tile data in the lane layout of the solver (rows 4g + r, col x).
+// Normalises one tile held in the solver's lane layout (lane (g, x) has
+// rows 4g + r, column x) into the padded input grid; returns sigma.
 __device__ __forceinline__ void tile_inputs(const float *d_tile_lane, const float *gm_lane,
-                                            float *in2, int lane, float &sigma) {
+                                            float *in_pad, int lane, float &sigma) {
   float red[2] = {d_tile_lane[0] + d_tile_lane[1] + (d_tile_lane[2] + d_tile_lane[3]),
                   gm_lane[0] + gm_lane[1] + (gm_lane[2] + gm_lane[3])};
   for (int off = 32; off >= 1; off >>= 1) {
@@ -205,96 +408,72 @@ __device__ __forceinline__ void tile_inputs(const float *d_tile_lane, const floa
   sigma = sqrtf(fmaxf(mu, 1.0f) + gmean);
   const float inv = 1.0f / sigma;
   const int x = lane & 15, g = lane >> 4;
+  in_pad_border(in_pad, lane);
 #pragma unroll
-  for (int r = 0; r < 4; r++) {
-    const int p = (4 * g + r) * 16 + x;
-    in2[p] = (d_tile_lane[r] - mu) * inv;
-    in2[256 + p] = gm_lane[r] * inv * inv;
-  }
+  for (int r = 0; r < 4; r++)
+    in_pad_put(in_pad, 4 * g + r, x, (d_tile_lane[r] - mu) * inv, gm_lane[r] * inv * inv);
 }
 
-// The network body on one tile whose inputs are in in2; the output y is left
-// in ybuf ([256] FP32, raster order).  acts (may be NULL): post-ReLU layer
-// 1-3 activations, BF16 [3][n][256][32], of tile t.
-__device__ __forceinline__ void net_body(const float *in2, __bf16 *bufA, __bf16 *bufB,
-                                         float *ybuf, const i4 *__restrict__ packs,
-                                         const float *__restrict__ P, int lane, __bf16 *acts,
-                                         long long n, long long t) {
-  const int x = lane & 15, g = lane >> 4;
-  // bias of the couts this lane writes: blk * 16 + 4g + r
-  float b1[2][4], b2[2][4], b3[2][4];
-#pragma unroll
-  for (int b = 0; b < 2; b++)
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-      b1[b][r] = P[OFF_B1 + 16 * b + 4 * g + r];
-      b2[b][r] = P[OFF_B2 + 16 * b + 4 * g + r];
-      b3[b][r] = P[OFF_B3 + 16 * b + 4 * g + r];
-    }
-  auto relu_store = [&](__bf16 *dst, float (&bias)[2][4], int layer) {
-    return [=, &bias](int y, int b, f4 acc) {
-      f4 v;
-      v.x = fmaxf(acc.x + bias[b][0], 0.0f);
-      v.y = fmaxf(acc.y + bias[b][1], 0.0f);
-      v.z = fmaxf(acc.z + bias[b][2], 0.0f);
-      v.w = fmaxf(acc.w + bias[b][3], 0.0f);
-      store4(dst, y, lane, b, v);
-      if (acts != nullptr) {
-        __bf16 *a = acts + (((long long)layer * n + t) * 256) * kCh;
-        store4(a, y, lane, b, v);
-      }
+// The network body on one tile whose inputs are in the padded grid `io`
+// (18 x 18 float pairs, then reused as layer 4's [16][16][3] partial sums);
+// returns the output y at pixels (4g + r, x) of lane (g, x).  acts (may be
+// NULL): post-ReLU layer 1-3 activations, BF16 [3][n][256][32] in the
+// standard channel order, of tile t.
+__device__ __forceinline__ f4 net_body(float *io, const i4 *__restrict__ packs,
+                                       const float *__restrict__ P, int lane, __bf16 *acts,
+                                       long long n, long long t) {
+  auto keep = [&](int layer) {
+    return [=](int y, i4 v) {
+      if (acts != nullptr)
+        store_row_std(acts + (((long long)layer * n + t) * 256) * kCh, y, lane, v);
     };
   };
-  // every phase reads what other lanes wrote in the previous one: the
-  // barriers order the LDS traffic (one wave per workgroup)
-  __syncthreads();
-  conv_small<2>(in2, packs + PK_L1 * 64, lane, relu_store(bufA, b1, 0));
-  __syncthreads();
-  conv32<2>(bufA, packs + PK_L2 * 64, lane, relu_store(bufB, b2, 1));
-  __syncthreads();
-  conv32<2>(bufB, packs + PK_L3 * 64, lane, relu_store(bufA, b3, 2));
-  __syncthreads();
-  const float b4 = P[OFF_B4];
-  conv32<1>(bufA, packs + PK_L4 * 64, lane, [&](int y, int, f4 acc) {
-    if (g == 0) ybuf[y * 16 + x] = acc.x + b4;
-  });
-  __syncthreads();
+  i4 act[16];
+  __syncthreads();  // the input grid is complete
+  layer_in2(io, opaque(packs + PK_L1 * 64), P + OFF_B1, lane, act, keep(0));
+  layer32(act, packs + PK_L2 * 64, P + OFF_B2, lane, keep(1));
+  layer32(act, packs + PK_L3 * 64, P + OFF_B3, lane, keep(2));
+  __syncthreads();  // every read of the grid precedes layer 4's writes over it
+  return layer_out1(act, opaque(packs + PK_L4T * 64), io, P[OFF_B4], lane);
 }
 
-constexpr int kNetLdsBytes = 2 * 256 * kCh * 2 + 3 * 256 * 4;  // bufA, bufB, in2, ybuf
+// forward LDS per tile: the input grid, then layer 4's partial sums
+constexpr int kNetIoFloats = 768;  // max(18 * 18 * 2, 16 * 16 * 3)
+// + the tile's data while the net runs (frame mode; each lane reads back
+// only what it wrote)
+constexpr int kNetFwdLdsBytes = (kNetIoFloats + 256) * 4;
+constexpr int kNetLdsBytes = 2 * 256 * kCh * 2 + 3 * 256 * 4;  // backward: two buffers, x, dY
 
 // Training / tensor mode: x [n][2][256] normalised inputs -> y [n][256].
-__global__ __launch_bounds__(64) void net_fwd_kernel(const float *__restrict__ xin,
-                                                     float *__restrict__ yout,
-                                                     __bf16 *__restrict__ acts,
-                                                     const i4 *__restrict__ packs,
-                                                     const float *__restrict__ P, long long n) {
+// One wave per tile, its activations in registers.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+void net_fwd_kernel(const float *__restrict__ xin, float *__restrict__ yout,
+                    __bf16 *__restrict__ acts, const i4 *__restrict__ packs,
+                    const float *__restrict__ P, long long n) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  __bf16 *bufA = reinterpret_cast<__bf16 *>(lds_raw);
-  __bf16 *bufB = bufA + 256 * kCh;
-  float *in2 = reinterpret_cast<float *>(bufB + 256 * kCh);
-  float *ybuf = in2 + 512;
+  float *io = reinterpret_cast<float *>(lds_raw);
   const int lane = threadIdx.x;
+  const int col = lane & 15, grp = lane >> 4;
   for (long long t = blockIdx.x; t < n; t += gridDim.x) {
-    for (int i = lane; i < 512; i += 64) in2[i] = xin[t * 512 + i];
-    net_body(in2, bufA, bufB, ybuf, packs, P, lane, acts, n, t);
-    for (int i = lane; i < 256; i += 64) yout[t * 256 + i] = ybuf[i];
+    in_pad_border(io, lane);
+    for (int i = lane; i < 256; i += 64)
+      in_pad_put(io, i >> 4, i & 15, xin[t * 512 + i], xin[t * 512 + 256 + i]);
+    const f4 y = net_body(io, packs, P, lane, acts, n, t);
+#pragma unroll
+    for (int r = 0; r < 4; r++) yout[t * 256 + (4 * grp + r) * 16 + col] = y[r];
     __syncthreads();
   }
 }
 
 // Inference mode over frames: gather each window exactly like the solver,
 // normalise, run the net and write u0 = d + sigma * y as [tiles][256].
-__global__ __launch_bounds__(64) void net_warm_kernel(const FrameDev *__restrict__ frames,
-                                                      int n_frames, long long total,
-                                                      float *__restrict__ u0,
-                                                      const i4 *__restrict__ packs,
-                                                      const float *__restrict__ P) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+void net_warm_kernel(const FrameDev *__restrict__ frames, int n_frames, long long total,
+                     float *__restrict__ u0, const i4 *__restrict__ packs,
+                     const float *__restrict__ P) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  __bf16 *bufA = reinterpret_cast<__bf16 *>(lds_raw);
-  __bf16 *bufB = bufA + 256 * kCh;
-  float *in2 = reinterpret_cast<float *>(bufB + 256 * kCh);
-  float *ybuf = in2 + 512;
+  float *io = reinterpret_cast<float *>(lds_raw);
+  float *dkeep = io + kNetIoFloats;
   const int lane = threadIdx.x;
   const int col = lane & 15, grp = lane >> 4;
   for (long long t = blockIdx.x; t < total; t += gridDim.x) {
@@ -331,13 +510,13 @@ __global__ __launch_bounds__(64) void net_warm_kernel(const FrameDev *__restrict
       }
     }
     float sigma;
-    tile_inputs(d, gm, in2, lane, sigma);
-    net_body(in2, bufA, bufB, ybuf, packs, P, lane, nullptr, 0, 0);
+    tile_inputs(d, gm, io, lane, sigma);
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-      const int p = (4 * grp + r) * 16 + col;
-      u0[t * 256 + p] = d[r] + sigma * ybuf[p];
-    }
+    for (int r = 0; r < 4; r++) dkeep[r * 64 + lane] = d[r];
+    const f4 y = net_body(io, packs, P, lane, nullptr, 0, 0);
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+      u0[t * 256 + (4 * grp + r) * 16 + col] = dkeep[r * 64 + lane] + sigma * y[r];
     __syncthreads();
   }
 }
@@ -517,10 +696,22 @@ hipError_t net_pack(const float *P, void *packs, hipStream_t s) {
 
 size_t net_packs_bytes() { return (size_t)N_PACKS * 512 * 2; }
 
+// Workgroups (one wave, one tile at a time) that fill the device: the
+// occupancy limit of the forward kernels times the CUs, at most n.
+int net_forward_grid(long long n, int cus) {
+  int blocks = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &blocks, reinterpret_cast<const void *>(net_warm_kernel), 64, kNetFwdLdsBytes) !=
+          hipSuccess ||
+      blocks < 1)
+    blocks = 4;
+  return (int)std::max<long long>(1, std::min<long long>(n, (long long)blocks * cus));
+}
+
 hipError_t net_forward(const float *x, float *y, void *acts, const void *packs, const float *P,
                        long long n, int grid, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(net_fwd_kernel, dim3(grid), dim3(64), kNetLdsBytes, s, x, y,
+  hipLaunchKernelGGL(net_fwd_kernel, dim3(grid), dim3(64), kNetFwdLdsBytes, s, x, y,
                      reinterpret_cast<__bf16 *>(acts), reinterpret_cast<const i4 *>(packs), P, n);
   return hipGetLastError();
 }
@@ -538,7 +729,7 @@ hipError_t net_backward(const float *x, const void *acts, const float *gout, flo
 hipError_t net_warm(const FrameDev *frames, int n_frames, long long total, float *u0,
                     const void *packs, const float *P, int grid, hipStream_t s) {
   if (total <= 0) return hipSuccess;
-  hipLaunchKernelGGL(net_warm_kernel, dim3(grid), dim3(64), kNetLdsBytes, s, frames, n_frames,
+  hipLaunchKernelGGL(net_warm_kernel, dim3(grid), dim3(64), kNetFwdLdsBytes, s, frames, n_frames,
                      total, u0, reinterpret_cast<const i4 *>(packs), P);
   return hipGetLastError();
 }
