@@ -110,13 +110,24 @@ __device__ __forceinline__ i4 dpp_i4(i4 v) {
   r.w = __builtin_amdgcn_update_dpp(0, v.w, CTRL, 0xF, 0xF, true);
   return r;
 }
-// The same pointer, opaque to the optimiser: loads through it are not hoisted
-// out of the per-tile loop (the operand packs of all four layers would not
-// fit in registers beside the working set).
-template <class T>
-__device__ __forceinline__ const T *opaque(const T *p) {
-  asm volatile("" : "+s"(p));
-  return p;
+// A zero the optimiser cannot see through.  A loop-invariant address offset
+// by it keeps its loads inside the per-tile loop (the operand packs of all
+// four layers would not fit in registers beside the working set), and the
+// address keeps its address space.
+__device__ __forceinline__ int opaque_zero() {
+  int z = 0;
+  asm volatile("" : "+s"(z));
+  return z;
+}
+
+// Orders one wave's LDS accesses across its lanes.  The LDS performs a
+// wave's instructions in issue order, so only the compiler has to be kept
+// from moving accesses across this point (wavefront-scope fences emit no
+// instructions).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // Accumulator start of the 16-channel block b: the bias of the output
@@ -193,18 +204,33 @@ __device__ __forceinline__ void conv32(const __bf16 *in, const i4 *__restrict__ 
   }
 }
 
+// Two floats as packed BF16 (round to nearest even), low half first.
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f2 __attribute__((ext_vector_type(2)));
+typedef short s2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ int bf16_pair(float lo, float hi) {
+  const f2 v = {lo, hi};
+  return __builtin_bit_cast(int, __builtin_convertvector(v, bf16x2));
+}
+
+// ReLU of a packed BF16 pair, clamping the halves as signed 16-bit integers
+// at 0: a BF16 with its sign bit set is a negative int16 and a positive one
+// keeps its order, so this equals rounding after the ReLU (negative values,
+// -0 included, become +0).
+__device__ __forceinline__ int relu_pair(float lo, float hi) {
+  const s2 z = {0, 0};
+  const s2 h = __builtin_bit_cast(s2, bf16_pair(lo, hi));
+  return __builtin_bit_cast(int, __builtin_elementwise_max(h, z));
+}
+
 // ReLU of one row's two block accumulators as a BF16 register row.
 __device__ __forceinline__ i4 relu_row(f4 a0, f4 a1) {
-  bf16x8 h;
-  h[0] = (__bf16)fmaxf(a0.x, 0.0f);
-  h[1] = (__bf16)fmaxf(a0.y, 0.0f);
-  h[2] = (__bf16)fmaxf(a0.z, 0.0f);
-  h[3] = (__bf16)fmaxf(a0.w, 0.0f);
-  h[4] = (__bf16)fmaxf(a1.x, 0.0f);
-  h[5] = (__bf16)fmaxf(a1.y, 0.0f);
-  h[6] = (__bf16)fmaxf(a1.z, 0.0f);
-  h[7] = (__bf16)fmaxf(a1.w, 0.0f);
-  return __builtin_bit_cast(i4, h);
+  i4 r;
+  r.x = relu_pair(a0.x, a0.y);
+  r.y = relu_pair(a0.z, a0.w);
+  r.z = relu_pair(a1.x, a1.y);
+  r.w = relu_pair(a1.z, a1.w);
+  return r;
 }
 
 // A register row written as row y of a [256][32] BF16 buffer in the
@@ -225,7 +251,7 @@ __device__ __forceinline__ int in_pad_offset(int tap, int x) {
 }
 
 template <class Keep>
-__device__ __forceinline__ void layer_in2(const float *in_pad, const i4 *__restrict__ packs,
+__device__ __forceinline__ void layer_in2(const float *in_pad, const i4 *packs,
                                           const float *bias, int lane, i4 (&act)[16],
                                           Keep keep) {
   const i4 A0 = packs[lane], A1 = packs[64 + lane];
@@ -236,14 +262,12 @@ __device__ __forceinline__ void layer_in2(const float *in_pad, const i4 *__restr
   for (int j = 0; j < 4; j++) off[j] = in_pad_offset(4 * g + j, x);
 #pragma unroll
   for (int y = 0; y < 16; y++) {
-    bf16x8 bv;
+    i4 B;
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       const float2 v = *reinterpret_cast<const float2 *>(in_pad + y * 36 + off[j]);
-      bv[2 * j] = (__bf16)v.x;
-      bv[2 * j + 1] = (__bf16)v.y;
+      B[j] = bf16_pair(v.x, v.y);
     }
-    const i4 B = __builtin_bit_cast(i4, bv);
     act[y] = relu_row(mfma_bf16(A0, B, c0), mfma_bf16(A1, B, c1));
     keep(y, act[y]);
   }
@@ -256,10 +280,11 @@ __device__ __forceinline__ void layer_in2(const float *in_pad, const i4 *__restr
 // held as values when rows y and y + 1 are overwritten; rows y + 1 and y + 2
 // carry over to the next step, so each row is shifted once per layer.
 template <class Keep>
-__device__ __forceinline__ void layer32(i4 (&act)[16], const i4 *__restrict__ packs_in,
-                                        const float *bias_in, int lane, Keep keep) {
-  const i4 *packs = opaque(packs_in);
-  const float *bias = opaque(bias_in);
+__device__ __forceinline__ void layer32(i4 (&act)[16], const i4 *packs_in, const float *bias_in,
+                                        int lane, Keep keep) {
+  const int z0 = opaque_zero();
+  const i4 *packs = packs_in + z0;
+  const float *bias = bias_in + z0;
   i4 A[2][9];
 #pragma unroll
   for (int b = 0; b < 2; b++)
@@ -319,7 +344,7 @@ __device__ __forceinline__ void layer32(i4 (&act)[16], const i4 *__restrict__ pa
 // and, after one barrier,
 //   out[y][x] = b4 + D_y[x - 1][0] + D_y[x][1] + D_y[x + 1][2]
 // (zero past the tile) for the pixels (4g + r, x) of lane (g, x), r = 0..3.
-__device__ __forceinline__ f4 layer_out1(const i4 (&act)[16], const i4 *__restrict__ packs,
+__device__ __forceinline__ f4 layer_out1(const i4 (&act)[16], const i4 *packs,
                                          float *dbuf, float b4, int lane) {
   const i4 B0 = packs[lane], B1 = packs[64 + lane], B2 = packs[128 + lane];
   const int n = lane & 15, g = lane >> 4;
@@ -333,7 +358,7 @@ __device__ __forceinline__ f4 layer_out1(const i4 (&act)[16], const i4 *__restri
 #pragma unroll
       for (int r = 0; r < 4; r++) dbuf[(y * 16 + 4 * g + r) * 3 + n] = d[r];
   }
-  __syncthreads();
+  wave_sync();
   f4 out;
 #pragma unroll
   for (int r = 0; r < 4; r++) {
@@ -416,67 +441,89 @@ __device__ __forceinline__ void tile_inputs(const float *d_tile_lane, const floa
 
 // The network body on one tile whose inputs are in the padded grid `io`
 // (18 x 18 float pairs, then reused as layer 4's [16][16][3] partial sums);
-// returns the output y at pixels (4g + r, x) of lane (g, x).  acts (may be
-// NULL): post-ReLU layer 1-3 activations, BF16 [3][n][256][32] in the
-// standard channel order, of tile t.
-__device__ __forceinline__ f4 net_body(float *io, const i4 *__restrict__ packs,
-                                       const float *__restrict__ P, int lane, __bf16 *acts,
-                                       long long n, long long t) {
+// returns the output y at pixels (4g + r, x) of lane (g, x).  pk / bl: the
+// forward packs and biases staged in LDS.  acts (may be NULL): post-ReLU
+// layer 1-3 activations, BF16 [3][n][256][32] in the standard channel order,
+// of tile t.  One wave runs it; io is the wave's own.
+__device__ __forceinline__ f4 net_body(float *io, const i4 *pk, const float *bl, int lane,
+                                       __bf16 *acts, long long n, long long t) {
   auto keep = [&](int layer) {
     return [=](int y, i4 v) {
       if (acts != nullptr)
         store_row_std(acts + (((long long)layer * n + t) * 256) * kCh, y, lane, v);
     };
   };
+  const int z = opaque_zero();
   i4 act[16];
-  __syncthreads();  // the input grid is complete
-  layer_in2(io, opaque(packs + PK_L1 * 64), P + OFF_B1, lane, act, keep(0));
-  layer32(act, packs + PK_L2 * 64, P + OFF_B2, lane, keep(1));
-  layer32(act, packs + PK_L3 * 64, P + OFF_B3, lane, keep(2));
-  __syncthreads();  // every read of the grid precedes layer 4's writes over it
-  return layer_out1(act, opaque(packs + PK_L4T * 64), io, P[OFF_B4], lane);
+  wave_sync();  // the input grid is complete
+  layer_in2(io, pk + (PK_L1 - PK_L1) * 64 + z, bl + z, lane, act, keep(0));
+  layer32(act, pk + (PK_L2 - PK_L1) * 64, bl + 32, lane, keep(1));
+  layer32(act, pk + (PK_L3 - PK_L1) * 64, bl + 64, lane, keep(2));
+  wave_sync();  // every read of the grid precedes layer 4's writes over it
+  return layer_out1(act, pk + (PK_L4T - PK_L1) * 64 + z, io, bl[96 + z], lane);
 }
 
-// forward LDS per tile: the input grid, then layer 4's partial sums
+// Forward workgroups: kNetWaves waves, each running its own tiles, share the
+// forward packs (layers 1-4, 41 KB) and biases staged once in LDS; each
+// wave owns an input grid / layer-4 buffer and a copy of its tile's data.
+constexpr int kNetWaves = 4;
+constexpr int kFwdPacks = PK_B4 - PK_L1;
 constexpr int kNetIoFloats = 768;  // max(18 * 18 * 2, 16 * 16 * 3)
-// + the tile's data while the net runs (frame mode; each lane reads back
-// only what it wrote)
-constexpr int kNetFwdLdsBytes = (kNetIoFloats + 256) * 4;
+constexpr int kNetWaveFloats = kNetIoFloats + 256;
+constexpr int kNetFwdLdsBytes = kFwdPacks * 1024 + 128 * 4 + kNetWaves * kNetWaveFloats * 4;
 constexpr int kNetLdsBytes = 2 * 256 * kCh * 2 + 3 * 256 * 4;  // backward: two buffers, x, dY
+
+// Copies the forward packs and the biases [b1 | b2 | b3 | b4] into LDS.
+__device__ __forceinline__ void stage_forward(const i4 *__restrict__ packs,
+                                              const float *__restrict__ P, i4 *pk, float *bl) {
+  for (int i = threadIdx.x; i < kFwdPacks * 64; i += blockDim.x) pk[i] = packs[PK_L1 * 64 + i];
+  for (int i = threadIdx.x; i < 97; i += blockDim.x)
+    bl[i] = i < 32 ? P[OFF_B1 + i]
+                   : (i < 64 ? P[OFF_B2 + i - 32] : (i < 96 ? P[OFF_B3 + i - 64] : P[OFF_B4]));
+  __syncthreads();
+}
 
 // Training / tensor mode: x [n][2][256] normalised inputs -> y [n][256].
 // One wave per tile, its activations in registers.
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ __launch_bounds__(64 * kNetWaves) __attribute__((amdgpu_waves_per_eu(2)))
 void net_fwd_kernel(const float *__restrict__ xin, float *__restrict__ yout,
                     __bf16 *__restrict__ acts, const i4 *__restrict__ packs,
                     const float *__restrict__ P, long long n) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  float *io = reinterpret_cast<float *>(lds_raw);
-  const int lane = threadIdx.x;
+  i4 *pk = reinterpret_cast<i4 *>(lds_raw);
+  float *bl = reinterpret_cast<float *>(pk + kFwdPacks * 64);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  float *io = bl + 128 + wave * kNetWaveFloats;
   const int col = lane & 15, grp = lane >> 4;
-  for (long long t = blockIdx.x; t < n; t += gridDim.x) {
+  stage_forward(packs, P, pk, bl);
+  for (long long t = (long long)blockIdx.x * kNetWaves + wave; t < n;
+       t += (long long)gridDim.x * kNetWaves) {
     in_pad_border(io, lane);
     for (int i = lane; i < 256; i += 64)
       in_pad_put(io, i >> 4, i & 15, xin[t * 512 + i], xin[t * 512 + 256 + i]);
-    const f4 y = net_body(io, packs, P, lane, acts, n, t);
+    const f4 y = net_body(io, pk, bl, lane, acts, n, t);
 #pragma unroll
     for (int r = 0; r < 4; r++) yout[t * 256 + (4 * grp + r) * 16 + col] = y[r];
-    __syncthreads();
+    wave_sync();
   }
 }
 
 // Inference mode over frames: gather each window exactly like the solver,
 // normalise, run the net and write u0 = d + sigma * y as [tiles][256].
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ __launch_bounds__(64 * kNetWaves) __attribute__((amdgpu_waves_per_eu(2)))
 void net_warm_kernel(const FrameDev *__restrict__ frames, int n_frames, long long total,
                      float *__restrict__ u0, const i4 *__restrict__ packs,
                      const float *__restrict__ P) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  float *io = reinterpret_cast<float *>(lds_raw);
+  i4 *pk = reinterpret_cast<i4 *>(lds_raw);
+  float *bl = reinterpret_cast<float *>(pk + kFwdPacks * 64);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  float *io = bl + 128 + wave * kNetWaveFloats;
   float *dkeep = io + kNetIoFloats;
-  const int lane = threadIdx.x;
   const int col = lane & 15, grp = lane >> 4;
-  for (long long t = blockIdx.x; t < total; t += gridDim.x) {
+  stage_forward(packs, P, pk, bl);
+  for (long long t = (long long)blockIdx.x * kNetWaves + wave; t < total;
+       t += (long long)gridDim.x * kNetWaves) {
     int lo = 0, hi = n_frames - 1;
     while (lo < hi) {
       int mid = (lo + hi + 1) >> 1;
@@ -513,11 +560,11 @@ void net_warm_kernel(const FrameDev *__restrict__ frames, int n_frames, long lon
     tile_inputs(d, gm, io, lane, sigma);
 #pragma unroll
     for (int r = 0; r < 4; r++) dkeep[r * 64 + lane] = d[r];
-    const f4 y = net_body(io, packs, P, lane, nullptr, 0, 0);
+    const f4 y = net_body(io, pk, bl, lane, nullptr, 0, 0);
 #pragma unroll
     for (int r = 0; r < 4; r++)
       u0[t * 256 + (4 * grp + r) * 16 + col] = dkeep[r * 64 + lane] + sigma * y[r];
-    __syncthreads();
+    wave_sync();
   }
 }
 
@@ -696,22 +743,24 @@ hipError_t net_pack(const float *P, void *packs, hipStream_t s) {
 
 size_t net_packs_bytes() { return (size_t)N_PACKS * 512 * 2; }
 
-// Workgroups (one wave, one tile at a time) that fill the device: the
-// occupancy limit of the forward kernels times the CUs, at most n.
+// Forward workgroups (kNetWaves waves, one tile per wave at a time) that fill
+// the device: the occupancy limit of the forward kernels times the CUs, at
+// most one tile per wave.
 int net_forward_grid(long long n, int cus) {
   int blocks = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-          &blocks, reinterpret_cast<const void *>(net_warm_kernel), 64, kNetFwdLdsBytes) !=
-          hipSuccess ||
+          &blocks, reinterpret_cast<const void *>(net_warm_kernel), 64 * kNetWaves,
+          kNetFwdLdsBytes) != hipSuccess ||
       blocks < 1)
-    blocks = 4;
-  return (int)std::max<long long>(1, std::min<long long>(n, (long long)blocks * cus));
+    blocks = 1;
+  const long long need = (n + kNetWaves - 1) / kNetWaves;
+  return (int)std::max<long long>(1, std::min<long long>(need, (long long)blocks * cus));
 }
 
 hipError_t net_forward(const float *x, float *y, void *acts, const void *packs, const float *P,
                        long long n, int grid, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(net_fwd_kernel, dim3(grid), dim3(64), kNetFwdLdsBytes, s, x, y,
+  hipLaunchKernelGGL(net_fwd_kernel, dim3(grid), dim3(64 * kNetWaves), kNetFwdLdsBytes, s, x, y,
                      reinterpret_cast<__bf16 *>(acts), reinterpret_cast<const i4 *>(packs), P, n);
   return hipGetLastError();
 }
@@ -729,7 +778,8 @@ hipError_t net_backward(const float *x, const void *acts, const float *gout, flo
 hipError_t net_warm(const FrameDev *frames, int n_frames, long long total, float *u0,
                     const void *packs, const float *P, int grid, hipStream_t s) {
   if (total <= 0) return hipSuccess;
-  hipLaunchKernelGGL(net_warm_kernel, dim3(grid), dim3(64), kNetFwdLdsBytes, s, frames, n_frames,
+  hipLaunchKernelGGL(net_warm_kernel, dim3(grid), dim3(64 * kNetWaves), kNetFwdLdsBytes, s,
+                     frames, n_frames,
                      total, u0, reinterpret_cast<const i4 *>(packs), P);
   return hipGetLastError();
 }

@@ -43,10 +43,15 @@ def test_forward_matches_oracle():
     err = numpy.max(numpy.abs(y.cpu().numpy() - yr))
     scale = numpy.max(numpy.abs(yr))
     _say("forward max abs err", err, "scale", scale)
-    assert err <= 1e-2 * scale
-    # plain FP32 torch conv as a second opinion (no bf16 rounding): looser
+    # the oracle rounds the same operands to BF16; what remains is FP32 vs
+    # FP64 accumulation (measured 1.7e-6 of the scale) and the odd BF16
+    # rounding-boundary flip of an activation it causes
+    assert err <= 1e-4 * scale
+    # the same net without BF16 rounding: what is left is the rounding itself
     yf, _ = ref.forward(p, x.cpu().numpy().astype(numpy.float64), bf16=False)
-    assert numpy.max(numpy.abs(y.cpu().numpy() - yf)) <= 5e-2 * scale
+    errf = numpy.max(numpy.abs(y.cpu().numpy() - yf))
+    _say("forward vs unrounded oracle max abs err", errf)
+    assert errf <= 5e-2 * scale
 
 
 def test_backward_matches_oracle():
@@ -67,7 +72,9 @@ def test_backward_matches_oracle():
             num = numpy.linalg.norm(gk[sl] - gr[sl])
             den = numpy.linalg.norm(gr[sl])
             _say("layer", li + 1, name, "rel err", num / den)
-            assert num <= 3e-2 * den, (li, name)
+            # measured at most 5.7e-3 (layer 1, whose gradient sums over
+            # the most BF16-rounded products)
+            assert num <= 1.5e-2 * den, (li, name)
 
 
 def test_learned_start_matches_plain_cost_and_tensor_path():
