@@ -117,6 +117,7 @@ struct ncs_ctx {
   std::vector<float> tab_host = std::vector<float>(kTableFloats, 0.0f);
   bool has_linear = false;
   float lin_c = 0.0f;
+  float k_norm = 0.0f;  // max row sum |2 q_ee| + |2 q_oo| of the mask tables
   // autotuned schedules, keyed by (general mask, padded geometry, FP16 split)
   std::map<TuneKey, Schedule> tuned;
   // last uploaded frame descriptors (pinned staging + completion event)
@@ -328,7 +329,7 @@ extern "C" int ncs_hip_set_otf_mask_window(ncs_ctx *ctx, const float *mask, int 
       qmax = std::max(qmax, std::fabs(qcs[k][l]));
     }
   std::vector<float> &tab = ctx->tab_host;
-  double qo_max = 0.0;
+  double qo_max = 0.0, k_norm = 0.0;
   for (int lane = 0; lane < 64; lane++) {
     int g = lane >> 4, x = lane & 15;
     for (int r = 0; r < 4; r++) {
@@ -353,6 +354,7 @@ extern "C" int ncs_hip_set_otf_mask_window(ncs_ctx *ctx, const float *mask, int 
       tab[lane * kTableLaneFloats + 8 + r] = (float)(2.0 * qee);
       tab[lane * kTableLaneFloats + 12 + r] = (float)(sgn * 2.0 * qoo);
       qo_max = std::max(qo_max, std::fabs(qoo));
+      k_norm = std::max(k_norm, 2.0 * std::fabs(qee) + 2.0 * std::fabs(qoo));
     }
   }
   HIPCHK(set_device(ctx));
@@ -360,6 +362,7 @@ extern "C" int ncs_hip_set_otf_mask_window(ncs_ctx *ctx, const float *mask, int 
   HIPCHK(hipMemcpy(ctx->tables.p, tab.data(), tab.size() * sizeof(float),
                    hipMemcpyHostToDevice));
   ctx->general = qo_max > 1e-12 * std::max(qmax, 1e-300);
+  ctx->k_norm = (float)k_norm;
   std::memset(ctx->mask_fft, 0, sizeof(ctx->mask_fft));
   std::memcpy(ctx->mask_fft, mask, (size_t)win * win * sizeof(float));
   ctx->win = win;
@@ -592,6 +595,7 @@ static SolveArgs make_args(const ncs_ctx *ctx, const FrameDev *d_frames, int n_f
   a.precond = p->precond;
   a.lin_start = p->start == NCS_START_LINEAR && u0 == nullptr;
   a.lin_c = ctx->lin_c;
+  a.k_norm = ctx->k_norm;
   return a;
 }
 

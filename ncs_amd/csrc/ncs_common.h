@@ -74,6 +74,7 @@ struct SolveArgs {
   int precond;                 // spectral initial inverse Hessian (else s.y / y.y)
   int lin_start;               // learned linear start (coefficients in the tables)
   float lin_c;                 // its DC bias
+  float k_norm;                // bound on the NC Hessian's norm (row sums of the mask tables)
 };
 
 }  // namespace ncs

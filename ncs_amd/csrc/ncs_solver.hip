@@ -126,8 +126,9 @@ __device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x)
 // v_log_f32 (log2) times ln 2, two instructions.  The libm-accurate logf
 // adds denormal rescaling and an extended-precision ln 2 product (about a
 // dozen instructions) that the solver's arguments (>= 1e-6) never need.
+constexpr float kLn2 = 0.693147182f, kInvLn2 = 1.44269504f;
 __device__ __forceinline__ float ln_hw(float x) {
-  return __builtin_amdgcn_logf(x) * 0.693147182f;
+  return __builtin_amdgcn_logf(x) * kLn2;
 }
 
 // log(1 + x) for x > -1, branch-free: the rounding error of u = 1 + x is
@@ -262,6 +263,14 @@ __device__ __forceinline__ void to_pix(const float (&w)[4], const TB &T, float (
 // the FP16 split (about 2^-22 per operand) to FP32: the default tolerance
 // (1e-5) stops at it, tighter tolerances finish in FP32.
 constexpr float kF16Floor = 1.0e-5f;
+
+// Bound on the error one incremental update adds to the spectral NC
+// gradient (K z += t K d), relative to |K| max_j |u_j|: the fma's rounding
+// (2^-24 of |K z| + |t K d|) and the mismatch between the spectral step t d
+// and the pixel step its basis change produced (below 1e-6 |t d| in FP32
+// and in the FP16 split), with |t d| = |u_new - u_old| <= 2 max_j |u_j|:
+// 2.2e-6, taken with a factor of about 2 to spare.
+constexpr float kNcDrift = 4.0e-6f;
 
 // [rn(x0..3), rn(x0..3 - hi)] as one 8-wide FP16 operand: 8 VALU.  The
 // packed conversions are compiler code on purpose: they are the first
@@ -934,6 +943,7 @@ void solve_kernel(SolveArgs A) {
 
     int status = NCS_REACHED_MAXITERS, iters = 0, evals = 1;
     bool gnc_exact = true;
+    int n_inc = 0;  // incremental NC-gradient updates since the last exact one
     float gg, uu, kappa;
     {
       float red[3] = {gs[0] * gs[0] + gs[1] * gs[1] + (gs[2] * gs[2] + gs[3] * gs[3]),
@@ -945,6 +955,7 @@ void solve_kernel(SolveArgs A) {
       kappa = red[2] * (1.0f / 256.0f);
     }
     bool done = gg <= A.eps * A.eps * fmaxf(1.0f, uu);
+    float uu_max = uu;  // largest |u|^2 of the iterates
     if (done) status = NCS_SUCCESS;
     else if (gg <= kF16Floor * kF16Floor * fmaxf(1.0f, uu)) exact = true;
 
@@ -1084,6 +1095,7 @@ void solve_kernel(SolveArgs A) {
         wave_sum<1>(red);
         gg = red[0];
         gnc_exact = true;
+        n_inc = 0;
         if (gg <= A.eps * A.eps * fmaxf(1.0f, uu)) {
           // the exact gradient already meets the stop rule at u, the point
           // the previous iteration accepted
@@ -1125,10 +1137,12 @@ void solve_kernel(SolveArgs A) {
       const bool moved = __any((ut[0] != u[0]) | (ut[1] != u[1]) | (ut[2] != u[2]) |
                                (ut[3] != u[3]));
       gnc_exact = false;
+      n_inc++;
       if ((k & 15) == 0) {  // periodic exact refresh of the NC gradient
         to_spec16<false>(ut, T, GB, st, zs, exact);
         nc_spec<GEN>(zs, T, gncn);
         gnc_exact = true;
+        n_inc = 0;
       }
       to_spec16(gt, T, GB, 1.0f, gts, exact);
       f += phi;
@@ -1151,15 +1165,25 @@ void solve_kernel(SolveArgs A) {
         wave_sum<3>(red);
         gg = red[0];
         uu = red[1];
+        uu_max = fmaxf(uu_max, uu);
         kappa = red[2] * (1.0f / 256.0f);
         yy = red[2];
         done = gg <= A.eps * A.eps * fmaxf(1.0f, uu);
         if (!done && gg <= kF16Floor * kF16Floor * fmaxf(1.0f, uu)) exact = true;
         if (!done || gnc_exact) break;
-        // confirm convergence with an exact NC gradient
+        // The incremental NC gradient differs from K u by rounding, at most
+        // kNcDrift |K| max_j |u_j| per update (|K| <= k_norm).  When that
+        // much drift, times alpha, cannot lift |g| past the stop rule's
+        // bound, an exact NC gradient would confirm convergence too;
+        // otherwise confirm with one.  (The FP16 build always confirms: the
+        // bound's registers cost it more spills than the refresh.)
+        if (!F16 && sqrtf(gg) + (float)n_inc * kNcDrift * alpha * A.k_norm * sqrtf(uu_max) <=
+                        A.eps * fmaxf(1.0f, sqrtf(uu)))
+          break;
         to_spec16<false>(ut, T, GB, st, zs, exact);
         nc_spec<GEN>(zs, T, gncn);
         gnc_exact = true;
+        n_inc = 0;
       }
       // history pair: s = t d, y = g_new - g (spectral)
       const bool added = ys > 0.0f && !done && moved && k != A.max_it;
