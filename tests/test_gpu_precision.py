@@ -37,13 +37,13 @@ def _tiles(seed, n):
     return d.astype(numpy.float64), g.astype(numpy.float64)
 
 
-def _check(ctx, d, g, params, label):
+def _check(ctx, d, g, params, label, alpha=0.1):
     mf = masks.to_fft_order(masks.otf_mask()).astype(numpy.float32)
     ctx.set_otf_mask(mf)
     d32, g32 = d.astype(numpy.float32), g.astype(numpy.float32)
-    u, it, st, cost = ctx.solve_tiles(d32, g32, 0.1, params)
-    ref = oracle.solve_tiles(d32, g32, mf, 0.1, f32=True)
-    f64, gn, un = oracle.eval_cost_f32(u, d32, g32, mf, 0.1)
+    u, it, st, cost = ctx.solve_tiles(d32, g32, alpha, params)
+    ref = oracle.solve_tiles(d32, g32, mf, alpha, f32=True)
+    f64, gn, un = oracle.eval_cost_f32(u, d32, g32, mf, alpha)
     rel = (f64 - ref["cost"]) / numpy.abs(ref["cost"])
     _say(label, "max rel cost", float(rel.max()), "iters", float(it.mean()),
          "status", dict(zip(*[x.tolist() for x in numpy.unique(st, return_counts=True)])),
@@ -61,6 +61,19 @@ def test_scaled_data_reach_the_oracle_minimiser(gpu_ctx, scale, precision):
     d, g = _tiles(11, 64)
     _check(gpu_ctx, d * scale, g * scale, _lib.default_params(precision=precision),
            "precision %d scale %g" % (precision, scale))
+
+
+@pytest.mark.parametrize("alpha", [2.0, 20.0])
+def test_strong_regularisation_still_meets_the_stop_rule(gpu_ctx, alpha):
+    """The FP32 build accepts convergence on its incremental NC gradient when
+    the gradient's rounding-drift bound, which grows with alpha, cannot
+    cross the stop rule, and confirms with an exact gradient otherwise.  At
+    large alpha the bound exceeds the margin: the FP64 gradient must still
+    meet the stop rule and the cost the oracle's."""
+    d, g = _tiles(17, 64)
+    for precision in (FP32, F16):
+        _check(gpu_ctx, d, g, _lib.default_params(precision=precision),
+               "alpha %g precision %d" % (alpha, precision), alpha=alpha)
 
 
 def test_overflowing_gamma_transform_falls_back_to_fp32(gpu_ctx):
