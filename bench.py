@@ -510,11 +510,11 @@ def make_uneven_frames():
     return frames
 
 
-def run_uneven(args, frames):
-    """The uneven batch; returns its JSON object."""
+def _uneven_device(frames):
+    """Context, device frames and descriptors of the uneven batch."""
     import torch
     from ncs_amd import _lib, masks
-    from ncs_amd.workloads import block_tiles, uneven_specs
+    from ncs_amd.workloads import uneven_specs
     shapes, alphas, seeds = uneven_specs()
     dev = torch.device("cuda:0")
     ctx = _lib.context(0)
@@ -528,8 +528,62 @@ def run_uneven(args, frames):
                  height=shapes[i][0], width=shapes[i][1], alpha=alphas[i], geometry=G)
             for i in range(len(d))]
     ntiles = [_lib.tile_count(h, w, G) for h, w in shapes]
-    total = int(sum(ntiles))
-    pix = int(sum(h * w for h, w in shapes))
+    return dict(ctx=ctx, mf=mf, G=G, d=d, g=g, out=out, desc=desc, shapes=shapes,
+                alphas=alphas, seeds=seeds, ntiles=ntiles, total=int(sum(ntiles)),
+                pix=int(sum(h * w for h, w in shapes)))
+
+
+def run_uneven_path(args, frames, history):
+    """One path of the uneven batch for profiler runs: the linear start at
+    the given history, 20 waves per CU and 4 per workgroup, without
+    autotuning or the oracle; warm-up, timed and one check launch."""
+    import torch
+    from ncs_amd import _lib, linstart
+    U = _uneven_device(frames)
+    ctx, desc, total = U["ctx"], U["desc"], U["total"]
+    if args.copy_probe:
+        a = torch.empty(1 << 28, dtype=torch.float32, device="cuda:0").fill_(1.0)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+    ctx.set_linear_start(linstart.load())
+    iters = torch.zeros(total, dtype=torch.int32, device="cuda:0")
+    status = torch.zeros_like(iters)
+    cost = torch.zeros(total, dtype=torch.float32, device="cuda:0")
+    p = _lib.default_params(max_iterations=MAX_IT, start=_lib.START_LINEAR, history=history,
+                            waves_per_cu=20, waves_per_group=4)
+    stream = torch.cuda.current_stream()
+    for _ in range(args.warmup):
+        ctx.reduce_frames_device(desc, p, None, iters.data_ptr(), status.data_ptr(), None,
+                                 stream.cuda_stream)
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(args.steps)]
+    for e0, e1 in ev:
+        e0.record(stream)
+        ctx.reduce_frames_device(desc, p, None, iters.data_ptr(), status.data_ptr(), None,
+                                 stream.cuda_stream)
+        e1.record(stream)
+    ctx.reduce_frames_device(desc, p, None, iters.data_ptr(), status.data_ptr(),
+                             cost.data_ptr(), stream.cuda_stream)
+    torch.cuda.synchronize()
+    return {"path": "uneven_linear", "history": history,
+            "kernel_ms": float(numpy.mean([a.elapsed_time(b) for a, b in ev])),
+            "iterations_mean": float(iters.cpu().numpy().mean()),
+            "source_hash": solver_source_hash()}
+
+
+def run_uneven(args, frames):
+    """The uneven batch; returns its JSON object."""
+    import torch
+    from ncs_amd import _lib
+    from ncs_amd.workloads import block_tiles
+    U = _uneven_device(frames)
+    ctx, mf, G, out, desc = U["ctx"], U["mf"], U["G"], U["out"], U["desc"]
+    shapes, alphas, seeds, ntiles = U["shapes"], U["alphas"], U["seeds"], U["ntiles"]
+    total, pix = U["total"], U["pix"]
+    dev = torch.device("cuda:0")
     iters = torch.zeros(total, dtype=torch.int32, device=dev)
     status = torch.zeros_like(iters)
     cost = torch.zeros(total, dtype=torch.float32, device=dev)
@@ -879,6 +933,9 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a 1 GiB device copy first (PMC byte-counter calibration)")
     ap.add_argument("--workload", choices=("batch", "uneven", "stream"), default="batch")
+    ap.add_argument("--uneven-history", type=int, default=None,
+                    help="with --workload uneven: launch only the linear path at this "
+                         "history and print its timing (profiler runs)")
     ap.add_argument("--stream-frames", type=int, default=200)
     ap.add_argument("--stream-depth", type=int, default=2,
                     help="captured pipelines the stream workload alternates between")
@@ -897,7 +954,11 @@ def main():
         sys.exit(2)
 
     if args.workload == "uneven":
-        print(json.dumps(run_uneven(args, make_uneven_frames())), flush=True)
+        if args.uneven_history is not None:
+            print(json.dumps(run_uneven_path(args, make_uneven_frames(), args.uneven_history)),
+                  flush=True)
+        else:
+            print(json.dumps(run_uneven(args, make_uneven_frames())), flush=True)
         return
     if args.workload == "stream":
         weights = None
