@@ -16,7 +16,8 @@
 // tile's activations stay in registers (16 rows x 4 VGPRs per lane, see
 // fwd_channel); the dx = -1 / +1 taps are DPP row_shr:1 / row_shl:1 of a
 // row, whose zero fill at the row end is exactly the zero padding of the
-// tile edge.  LDS holds only the input grid and layer 4's partial sums.
+// tile edge.  LDS holds the forward weight packs and biases, staged once
+// per workgroup, and per wave the input grid / layer 4's partial sums.
 //
 // Backward: the data gradient is the same convolution with flipped,
 // transposed weight packs; the weight gradient is a GEMM over pixels
