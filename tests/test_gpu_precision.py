@@ -37,8 +37,9 @@ def _tiles(seed, n):
     return d.astype(numpy.float64), g.astype(numpy.float64)
 
 
-def _check(ctx, d, g, params, label, alpha=0.1):
-    mf = masks.to_fft_order(masks.otf_mask()).astype(numpy.float32)
+def _check(ctx, d, g, params, label, alpha=0.1, mask=None):
+    m = masks.otf_mask() if mask is None else mask
+    mf = masks.to_fft_order(m).astype(numpy.float32)
     ctx.set_otf_mask(mf)
     d32, g32 = d.astype(numpy.float32), g.astype(numpy.float32)
     u, it, st, cost = ctx.solve_tiles(d32, g32, alpha, params)
@@ -63,6 +64,20 @@ def test_scaled_data_reach_the_oracle_minimiser(gpu_ctx, scale, precision):
            "precision %d scale %g" % (precision, scale))
 
 
+def _rotated_mask(angle=0.5, sa=3.0, sb=6.0):
+    """1 - an anisotropic Gaussian low-pass rotated by `angle`, made exactly
+    point-symmetric (m(k) = m(-k) on the 16-point grid, Nyquist included);
+    centred 16 x 16 layout, values in [0, 1]."""
+    k = numpy.fft.fftfreq(16, 1.0 / 16)
+    ky, kx = numpy.meshgrid(k, k, indexing="ij")
+    a = numpy.cos(angle) * kx + numpy.sin(angle) * ky
+    b = -numpy.sin(angle) * kx + numpy.cos(angle) * ky
+    m = 1.0 - numpy.exp(-(a / sa) ** 2 - (b / sb) ** 2)  # FFT order
+    neg = (-numpy.arange(16)) % 16
+    m = 0.5 * (m + m[neg][:, neg])
+    return numpy.fft.fftshift(m)
+
+
 @pytest.mark.parametrize("alpha", [2.0, 20.0])
 def test_strong_regularisation_still_meets_the_stop_rule(gpu_ctx, alpha):
     """The FP32 build accepts convergence on its incremental NC gradient when
@@ -71,9 +86,14 @@ def test_strong_regularisation_still_meets_the_stop_rule(gpu_ctx, alpha):
     large alpha the bound exceeds the margin: the FP64 gradient must still
     meet the stop rule and the cost the oracle's."""
     d, g = _tiles(17, 64)
-    for precision in (FP32, F16):
-        _check(gpu_ctx, d, g, _lib.default_params(precision=precision),
-               "alpha %g precision %d" % (alpha, precision), alpha=alpha)
+    # the OTF mask, and a rotated anisotropic one: point-symmetric but not
+    # even along each axis, so its odd-odd part couples coefficient pairs
+    # (the general build, whose |K| bound includes the coupling)
+    for name, mask in (("otf", None), ("rotated", _rotated_mask())):
+        for precision in (FP32, F16):
+            _check(gpu_ctx, d, g, _lib.default_params(precision=precision),
+                   "alpha %g %s mask precision %d" % (alpha, name, precision), alpha=alpha,
+                   mask=mask)
 
 
 def test_overflowing_gamma_transform_falls_back_to_fp32(gpu_ctx):
