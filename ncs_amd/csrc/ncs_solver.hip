@@ -383,14 +383,30 @@ __device__ __forceinline__ void apply_k(const float (&x)[4], const TB &T, float 
   to_pix(w, T, out);
 }
 
-// x <- H0 x in the spectral basis, H0 = diag(1 / (kappa + alpha qe)): the
-// inverse of the Hessian with the likelihood part replaced by its tile mean
-// kappa and the mask by its even-even part.
-template <class TB>
+// x <- H0 x in the spectral basis, H0 = (kappa I + alpha K)^-1: the inverse
+// of the Hessian with the likelihood part replaced by its tile mean kappa.
+// Without an odd-odd part K is the diagonal 2 qe; with one (GEN) it couples
+// each coefficient with one partner (lane ^ 40, same slot) through the
+// symmetric weight qo, and H0 inverts each 2 x 2 block
+//   [kappa + alpha qe, alpha qo; alpha qo, kappa + alpha qe'].
+template <bool GEN, class TB>
 __device__ __forceinline__ void h0_spec(float (&x)[4], const TB &T, float kappa, float alpha) {
   const f4 q = T.qe();
+  if constexpr (!GEN) {
 #pragma unroll
-  for (int r = 0; r < 4; r++) x[r] *= rcpf(fmaf(alpha, q[r], kappa));
+    for (int r = 0; r < 4; r++) x[r] *= rcpf(fmaf(alpha, q[r], kappa));
+  } else {
+    const f4 qo = T.qo();
+    float y[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const float xp = __shfl_xor(x[r], 40), qp = __shfl_xor(q[r], 40);
+      const float a = fmaf(alpha, q[r], kappa), d = fmaf(alpha, qp, kappa), b = alpha * qo[r];
+      y[r] = fmaf(d, x[r], -b * xp) * rcpf(fmaf(a, d, -b * b));
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) x[r] = y[r];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -972,7 +988,7 @@ void solve_kernel(SolveArgs A) {
     // liblbfgs's scalar H0 from the same point.
     bool spec_h0 = A.precond != 0;
     if (spec_h0 && !done) {
-      h0_spec(ds, T, kappa, alpha);
+      h0_spec<GEN>(ds, T, kappa, alpha);
       step = 1.0f;
     }
     float ys_last = 1.0f, yy_last = 1.0f;
@@ -999,7 +1015,7 @@ void solve_kernel(SolveArgs A) {
 #pragma unroll
         for (int r = 0; r < 4; r++) ds[r] = -gs[r];
         if (spec_h0) {
-          h0_spec(ds, T, kappa, alpha);
+          h0_spec<GEN>(ds, T, kappa, alpha);
         } else {
           const float sc = rsqrtf(gg);
 #pragma unroll
@@ -1030,9 +1046,10 @@ void solve_kernel(SolveArgs A) {
             ls = LS_ACCEPT;  // what the full search decides on this trial
             dphi_acc = dphi;
           } else {
-            // Full More-Thuente search (about once per tile).  Its state is
-            // kept in LDS and the memory clobbers stop the compiler from
-            // holding it in VGPRs across the vector trials.
+            // Full More-Thuente search (rare: on the 10 x 2048^2 benchmark
+            // batch every first trial is accepted).  Its state is kept in
+            // LDS and the memory clobbers stop the compiler from holding it
+            // in VGPRs across the vector trials.
             MtState &S_ = mt_lds;
             mt_init(S_, dg0, A);
             S_.stmin = 0.0f;
@@ -1104,7 +1121,7 @@ void solve_kernel(SolveArgs A) {
           break;
         }
         if (spec_h0) {
-          h0_spec(ds, T, kappa, alpha);
+          h0_spec<GEN>(ds, T, kappa, alpha);
           step = 1.0f;
         } else {
           step = rsqrtf(gg);
@@ -1233,7 +1250,7 @@ void solve_kernel(SolveArgs A) {
         }
       }
       if (spec_h0) {
-        h0_spec(ds, T, kappa, alpha);
+        h0_spec<GEN>(ds, T, kappa, alpha);
       } else {  // liblbfgs: H0 = (s.y / y.y) I
         const float hscale = (nh > 0) ? ys_last * rcpf(yy_last) : rsqrtf(gg);
 #pragma unroll
