@@ -52,6 +52,7 @@ def _check(ctx, d, g, params, label, alpha=0.1, mask=None):
     assert numpy.all(st == 0), label
     assert rel.max() <= 1e-5, label
     assert numpy.all(gn <= 2e-5 * numpy.maximum(1.0, un)), label
+    return it
 
 
 @pytest.mark.parametrize("precision", [FP32, F16])
@@ -91,9 +92,13 @@ def test_strong_regularisation_still_meets_the_stop_rule(gpu_ctx, alpha):
     # (the general build, whose |K| bound includes the coupling)
     for name, mask in (("otf", None), ("rotated", _rotated_mask())):
         for precision in (FP32, F16):
-            _check(gpu_ctx, d, g, _lib.default_params(precision=precision),
-                   "alpha %g %s mask precision %d" % (alpha, name, precision), alpha=alpha,
-                   mask=mask)
+            it = _check(gpu_ctx, d, g, _lib.default_params(precision=precision),
+                        "alpha %g %s mask precision %d" % (alpha, name, precision),
+                        alpha=alpha, mask=mask)
+            if name == "rotated":
+                # H0 inverts the coupled 2 x 2 blocks: about 3.5 iterations
+                # (12-17 with the diagonal alone)
+                assert it.mean() <= 6.0, (name, alpha, precision, it.mean())
 
 
 def test_overflowing_gamma_transform_falls_back_to_fp32(gpu_ctx):
