@@ -85,10 +85,12 @@ typedef struct {
   int fallback;        /* warm starts: keep u0 = data where f(data) <= f(u0) */
   int waves_per_cu;    /* persistent grid size; 0 = autotuned (else the
                           occupancy limit) */
-  int precond;         /* 1: initial inverse Hessian H0 = R^T diag(1/(kappa +
-                          alpha*2q)) R in the real DFT basis R, kappa = tile
-                          mean of (d+gamma)/(u+gamma)^2 (default); 0: the
-                          scalar s.y/y.y of liblbfgs and ncs.cl:1123 */
+  int precond;         /* 1: initial inverse Hessian H0 = (kappa I +
+                          alpha K)^-1 in the real DFT basis, kappa = tile
+                          mean of (d+gamma)/(u+gamma)^2 (default): diagonal
+                          for masks without an odd-odd part, 2 x 2 blocks
+                          of coupled coefficients otherwise; 0: the scalar
+                          s.y/y.y of liblbfgs and ncs.cl:1123 */
   int waves_per_group; /* tiles (waves) per workgroup, 1..4; 0 = autotuned
                           (else 1) */
   int precision;       /* NCS_PRECISION_* */
