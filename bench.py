@@ -52,6 +52,7 @@ HBM_PEAK_GBS = 8000.0
 SIMDS, CLOCK_GHZ = 256 * 4, 2.4  # MI355X: 256 CUs of 4 SIMDs, peak engine clock
 PCIE_GBS = 64.0  # PCIe 5.0 x16, one direction
 N_FRAMES, H, W, ALPHA, HISTORY, MAX_IT = 10, 2048, 2048, 0.1, 5, 200
+CORR_H0 = "curvature-corrected (precond=2)"
 
 
 def _frame(seed):
@@ -287,18 +288,25 @@ def run_gpu(frames_np, args, weights, lin_coef=None):
               file=sys.stderr, flush=True)
 
     # schedule: grid and workgroup size tuned on the whole batch (history
-    # stays m = 5, the workload's)
+    # stays m = 5, the workload's).  The m = 5 runs use the spectral H0
+    # (precond 1): at this depth the curvature correction's two basis changes
+    # per direction cost more than the iterations it saves.
     sched = None
     if args.schedule_cache and os.path.exists(args.schedule_cache):
         with open(args.schedule_cache) as fh:
             sched = json.load(fh)
+        if args.history is not None:  # profiler runs of another history build
+            for k in ("tuned", "linear", "f16"):
+                if sched.get(k):
+                    sched[k] = dict(sched[k], history=args.history)
     if sched is None:
-        tune = ctx.autotune(desc, _lib.default_params(history=HISTORY, max_iterations=MAX_IT),
+        tune = ctx.autotune(desc, _lib.default_params(history=HISTORY, max_iterations=MAX_IT,
+                                                      precond=1),
                             tune_history=False)
     else:
         tune = sched["m5"]
     print("[bench] autotune: %s" % tune, file=sys.stderr, flush=True)
-    plain = _lib.default_params(history=HISTORY, max_iterations=MAX_IT,
+    plain = _lib.default_params(history=HISTORY, max_iterations=MAX_IT, precond=1,
                                 waves_per_cu=tune["waves_per_cu"],
                                 waves_per_group=tune["waves_per_group"])
     timed(plain, "plain", args.steps, args.warmup)
@@ -387,7 +395,8 @@ def run_gpu(frames_np, args, weights, lin_coef=None):
         results["learned"]["fallback_rate"] = float(numpy.mean(c_dat <= c_net))
         results["learned"]["start_gain_rel"] = float(numpy.median((c_dat - c_net) / numpy.abs(c_dat)))
     if not args.quick:
-        fixed = _lib.default_params(history=HISTORY, max_iterations=MAX_IT, epsilon=0.0)
+        fixed = _lib.default_params(history=HISTORY, max_iterations=MAX_IT, epsilon=0.0,
+                                    precond=1)
         timed(fixed, "fixed200", max(1, min(args.steps, 2)), 1)
     return results, ntile_frame, total, ctx.info()
 
@@ -933,6 +942,9 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a 1 GiB device copy first (PMC byte-counter calibration)")
     ap.add_argument("--workload", choices=("batch", "uneven", "stream"), default="batch")
+    ap.add_argument("--history", type=int, default=None,
+                    help="with --schedule-cache: run the tuned, linear and FP16-split paths at "
+                         "this history instead of the cached one (profiler runs)")
     ap.add_argument("--uneven-history", type=int, default=None,
                     help="with --workload uneven: launch only the linear path at this "
                          "history and print its timing (profiler runs)")
@@ -1055,6 +1067,7 @@ def main():
                       "ms_per_step": round(T["wall_s"] * 1e3, 3),
                       "kernel_ms": round(T["kernel_ms"], 3),
                       "history": res["tune_history"]["history"],
+                      "initial_hessian": CORR_H0,
                       "waves_per_cu": res["tune_history"]["waves_per_cu"],
                       "waves_per_group": res["tune_history"]["waves_per_group"],
                       "iterations_mean": round(float(T["iters"].mean()), 3),
@@ -1072,9 +1085,12 @@ def main():
             "value": round(lm, 2), "unit": "Mpix/s", "ms_per_step": round(L["wall_s"] * 1e3, 3),
             "kernel_ms": round(L["kernel_ms"], 3),
             "history": TL["history"], "waves_per_cu": TL["waves_per_cu"],
-            "waves_per_group": TL["waves_per_group"],
+            "waves_per_group": TL["waves_per_group"], "initial_hessian": CORR_H0,
             "iterations_mean": round(float(L["iters"].mean()), 3),
             "iterations_saved_vs_plain_m5": round(float(plain["iters"].mean() - L["iters"].mean()), 3),
+            "iterations_saved_vs_tuned_plain_start": (
+                round(float(res["tuned"]["iters"].mean() - L["iters"].mean()), 3)
+                if "tuned" in res else None),
             "cost_vs_plain_max_rel": float(rel.max()),
             "cost_vs_plain_p99_rel": float(numpy.percentile(rel, 99)),
             "matched_1e5": bool(rel.max() <= 1e-5),
@@ -1095,7 +1111,7 @@ def main():
                     "near the stop rule and on overflow",
             "ms_per_step": round(F["wall_s"] * 1e3, 3), "kernel_ms": round(F["kernel_ms"], 3),
             "history": TF["history"], "waves_per_cu": TF["waves_per_cu"],
-            "waves_per_group": TF["waves_per_group"],
+            "waves_per_group": TF["waves_per_group"], "initial_hessian": CORR_H0,
             "iterations_mean": round(float(F["iters"].mean()), 3),
             "cost_vs_plain_max_rel": float(rel.max()),
             "matched_1e5": bool(rel.max() <= 1e-5),
@@ -1119,7 +1135,11 @@ def main():
                                            / (L.get("net_ms", float("nan")) * 1e-3)
                                            / 2.5e15, 4),
             "iterations_mean": round(float(L["iters"].mean()), 3),
+            "initial_hessian": CORR_H0,
             "iterations_saved": round(float(plain["iters"].mean() - L["iters"].mean()), 3),
+            "iterations_saved_vs_tuned_plain_start": (
+                round(float(res["tuned"]["iters"].mean() - L["iters"].mean()), 3)
+                if "tuned" in res else None),
             "fallback_rate_frame1": round(L.get("fallback_rate", float("nan")), 4),
             "median_start_cost_gain_rel": L.get("start_gain_rel"),
             "cost_vs_plain_max_rel": float(rel.max()),
@@ -1159,12 +1179,15 @@ def main():
                                   HISTORY),
                    "frames": len(frames_np), "height": H, "width": W, "alpha": ALPHA,
                    "history": hist_used, "reference_history": HISTORY,
+                   "initial_hessian": ("spectral (precond=1)" if start_used == "plain"
+                                       else CORR_H0),
                    "max_iterations": MAX_IT, "epsilon": 1e-5,
                    "mode": "converged (stop rule), at most %d iterations" % MAX_IT,
                    "precision": "fp32", "tiles": int(total), "path": start_used},
         "plain": {"value": round(mpix, 2), "unit": "Mpix/s",
                   "ms_per_step": round(plain["wall_s"] * 1e3, 3),
                   "kernel_ms": round(plain["kernel_ms"], 3), "history": HISTORY,
+                  "iterations_mean": round(float(plain["iters"].mean()), 3),
                   "initial_hessian": "spectral (precond=1)"},
         "plain_scalar_h0": _scalar_summary(res, plain, pix),
         "tuned": tuned_info,

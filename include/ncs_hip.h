@@ -85,11 +85,17 @@ typedef struct {
   int fallback;        /* warm starts: keep u0 = data where f(data) <= f(u0) */
   int waves_per_cu;    /* persistent grid size; 0 = autotuned (else the
                           occupancy limit) */
-  int precond;         /* 1: initial inverse Hessian H0 = (kappa I +
-                          alpha K)^-1 in the real DFT basis, kappa = tile
-                          mean of (d+gamma)/(u+gamma)^2 (default): diagonal
-                          for masks without an odd-odd part, 2 x 2 blocks
-                          of coupled coefficients otherwise; 0: the scalar
+  int precond;         /* initial inverse Hessian H0 of the two-loop
+                          recursion.  1: P = (kappa I + alpha K)^-1 in the
+                          real DFT basis, kappa = tile mean of the
+                          likelihood curvature (d+gamma)/(u+gamma)^2:
+                          diagonal for masks without an odd-odd part, 2 x 2
+                          blocks of coupled coefficients otherwise.
+                          2 (default): P - P D P, D = the per-pixel
+                          curvature less kappa (clamped at kappa / 2 from
+                          above), which adds the curvature's variation over
+                          the tile to first order (two more basis changes
+                          per direction, fewer iterations).  0: the scalar
                           s.y/y.y of liblbfgs and ncs.cl:1123 */
   int waves_per_group; /* tiles (waves) per workgroup, 1..4; 0 = autotuned
                           (else 1) */

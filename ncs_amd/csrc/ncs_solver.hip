@@ -409,6 +409,63 @@ __device__ __forceinline__ void h0_spec(float (&x)[4], const TB &T, float kappa,
   }
 }
 
+// Pixel curvatures above (1 + kH0Clamp) kappa count as that much in the
+// curvature-corrected H0.
+constexpr float kH0Clamp = 0.5f;
+
+// The curvature correction applies while the NC curvature, at most
+// alpha |K|, is within kH0CorrRatio of the mean likelihood curvature kappa.
+// Beyond that the likelihood alone holds the low frequencies, where P is
+// 1 / kappa and the curvature varies by more than kappa over the tile: the
+// first Neumann term is then a poor step there, and the gradient rule,
+// scaled by that small curvature, does not see the error.  Measured in
+// FP64: photon tiles at alpha 0.1 have ratios 5-11 and save one iteration in
+// three; at alpha 2 (ratios 110-215) the plain start took more iterations
+// with it, and data scaled by 1e4 (ratios near 1e5) stopped 3e-3 above the
+// minimiser's cost.  The spectral H0 is used beyond the ratio, decided per
+// tile from kappa at its start.
+constexpr float kH0CorrRatio = 30.0f;
+
+// x <- H0 x with the likelihood curvature's variation over the tile taken
+// to first order (precond 2).  The Hessian is P^-1 + D, P = (kappa I +
+// alpha K)^-1 (spectral, h0_spec) and D = diag(c_i - kappa) (pixels, c_i =
+// c / max(u + gamma, FITMIN)^2); H0 = P - P D P is the first Neumann term of
+// (P^-1 + D)^-1 = P (I + D P)^-1, applied with one basis change each way.
+// D is clamped at kH0Clamp kappa from above, which keeps H0 positive
+// definite: K is positive semidefinite, so P <= I / kappa and
+// P^1/2 D P^1/2 <= kH0Clamp I.  Only the preconditioner depends on this
+// approximation, not the minimiser.  iv is 1 / max(u + gamma, FITMIN) when
+// CACHED, else recomputed from u; s scales the transform to pixels like the
+// search direction's.
+template <bool GEN, bool CACHED, class TT, class GB>
+__device__ __forceinline__ void h0_corr(float (&x)[4], const TT &T, const GB &G,
+                                        const float (&c)[4], const float (&iv)[4],
+                                        const float (&u)[4], const float (&gm)[4], float kappa,
+                                        float alpha, float s, bool exact, float4 *stash) {
+  h0_spec<GEN>(x, T, kappa, alpha);
+  float xp[4], w[4];
+  // P x waits in LDS across the two basis changes (registers are the
+  // scarcer resource there); the clobbers keep the compiler from
+  // forwarding the stored value instead of reloading it
+  *stash = make_float4(x[0], x[1], x[2], x[3]);
+  asm volatile("" ::: "memory");
+  to_pix16(x, T, G, s, xp, exact);
+  const float hi = (1.0f + kH0Clamp) * kappa;
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const float v = CACHED ? iv[r] : rcpf(fmaxf(u[r] + gm[r], kFitMin));
+    xp[r] *= fminf(c[r] * v * v, hi) - kappa;
+  }
+  to_spec16(xp, T, G, 1.0f, w, exact);
+  h0_spec<GEN>(w, T, kappa, alpha);
+  asm volatile("" ::: "memory");
+  const float4 z = *stash;
+  x[0] = z.x - w[0];
+  x[1] = z.y - w[1];
+  x[2] = z.z - w[2];
+  x[3] = z.w - w[3];
+}
+
 // ---------------------------------------------------------------------------
 // More-Thuente line search on uniform scalars (More & Thuente 1994; the
 // liblbfgs default, lbfgs.h:704-719).  phi(t) = f(u + t d) - f(u).
@@ -738,9 +795,14 @@ void solve_kernel(SolveArgs A) {
   __shared__ MtState mt_state[kMaxWavesPerGroup];  // line-search state per wave
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform
   constexpr bool kCacheIv = M <= 3;  // keep 1/(u+gamma) across iterations
+  // The curvature-corrected H0 is built for histories up to 3; above that its
+  // two basis changes per direction cost more than the iterations it saves
+  // (measured at history 5), and precond 2 runs as precond 1.
+  constexpr bool kCorr = M <= 3;
   MtState &mt_lds = mt_state[wave];
-  float4 *const Sb = lds_hist + wave * (2 * M * 64);  // s vectors, ring of M
-  float4 *const Yb = Sb + M * 64;                     // y vectors, ring of M
+  float4 *const Sb = lds_hist + wave * ((2 * M + 1) * 64);  // s vectors, ring of M
+  float4 *const Yb = Sb + M * 64;                           // y vectors, ring of M
+  float4 *const Hz = Yb + M * 64 + (threadIdx.x & 63);      // this lane's H0 stash
   const int lane = threadIdx.x & 63;
   const int col = lane & 15, grp = lane >> 4;
 
@@ -985,10 +1047,16 @@ void solve_kernel(SolveArgs A) {
     // mean.  From a start far from the solution that mean can change by
     // orders of magnitude along the way and the line search then fail; a
     // tile whose search fails under it drops its history and continues with
-    // liblbfgs's scalar H0 from the same point.
+    // liblbfgs's scalar H0 from the same point.  A search that fails under
+    // the curvature-corrected H0 restarts the tile with the spectral one.
     bool spec_h0 = A.precond != 0;
+    // curvature-corrected H0 (uniform), decided per tile at its start
+    bool corr_h0 = kCorr && A.precond == 2 && alpha * A.k_norm <= kH0CorrRatio * kappa;
     if (spec_h0 && !done) {
-      h0_spec<GEN>(ds, T, kappa, alpha);
+      if (kCorr && corr_h0)
+        h0_corr<GEN, true>(ds, T, GB, c, iv, u, gm, kappa, alpha, st, exact, Hz);
+      else
+        h0_spec<GEN>(ds, T, kappa, alpha);
       step = 1.0f;
     }
     float ys_last = 1.0f, yy_last = 1.0f;
@@ -1087,35 +1155,51 @@ void solve_kernel(SolveArgs A) {
           }
         }
       }
-      if (ls != LS_ACCEPT && !exact) {
-        // The FP16 split's error can stall the search on an ill-conditioned
-        // tile, where the likelihood and NC parts of phi'(t) nearly cancel:
-        // the tile goes to FP32 basis changes, drops its history, refreshes
-        // its spectral gradient exactly and searches again from u.
+      // A failed search that the tile can still recover from refreshes its
+      // state exactly (FP32 basis changes, history dropped) and searches
+      // again:
+      //  * under the curvature-corrected H0, from the plain start (the data)
+      //    with the spectral H0: the corrected steps may have led where no
+      //    H0 recovers (on stiff masks, pixels at the FITMIN guard, where
+      //    the gradient is not the cost's derivative); the cost reference
+      //    is cleared and the iterations keep counting.  The fixed-iteration
+      //    mode (eps 0) ends in failed searches by design and skips this;
+      //  * in the FP16 split, from u in FP32: the split's error can stall
+      //    the search on an ill-conditioned tile, where the likelihood and NC
+      //    parts of phi'(t) nearly cancel.
+      const bool restart = kCorr && corr_h0 && A.eps > 0.0f;
+      if (ls != LS_ACCEPT && (restart || !exact)) {
+        if (restart) {
+          float nw[4] = {0.f, 0.f, 0.f, 0.f}, ng[4];
+          load_tile(F, G, t, u, ng, nw);
+          f = 0.0f;
+        }
+        corr_h0 = false;
         exact = true;
         nh = 0;
+        float gp[4];
         to_spec16<false>(u, T, GB, st, zs, exact);
         nc_spec<GEN>(zs, T, gncs);
-        float gp[4];
-#pragma unroll
-        for (int r = 0; r < 4; r++)
-          gp[r] = (PAD ? lin[r] : 1.0f) -
-                  c[r] * (kCacheIv ? iv[r] : rcpf(fmaxf(u[r] + gm[r], kFitMin)));
+        ll_grad_lane(u, gm, c, lin, gp, iv);
         to_spec16(gp, T, GB, 1.0f, gs, exact);
-        float red[1] = {0.0f};
 #pragma unroll
         for (int r = 0; r < 4; r++) {
           gs[r] = fmaf(alpha, gncs[r], gs[r]);
           ds[r] = -gs[r];
-          red[0] = fmaf(gs[r], gs[r], red[0]);
         }
-        wave_sum<1>(red);
+        float red[3] = {gs[0] * gs[0] + gs[1] * gs[1] + (gs[2] * gs[2] + gs[3] * gs[3]),
+                        u[0] * u[0] + u[1] * u[1] + (u[2] * u[2] + u[3] * u[3]),
+                        curv_lane(c, iv)};
+        wave_sum<3>(red);
         gg = red[0];
+        uu = red[1];
+        uu_max = fmaxf(uu_max, uu);
+        kappa = red[2] * (1.0f / 256.0f);
         gnc_exact = true;
         n_inc = 0;
         if (gg <= A.eps * A.eps * fmaxf(1.0f, uu)) {
-          // the exact gradient already meets the stop rule at u, the point
-          // the previous iteration accepted
+          // the exact gradient already meets the stop rule at u (the point
+          // the previous iteration accepted, or the plain start)
           iters = k - 1;
           status = NCS_SUCCESS;
           break;
@@ -1249,7 +1333,9 @@ void solve_kernel(SolveArgs A) {
           ds[3] = fmaf(-al[j], yv.w, ds[3]);
         }
       }
-      if (spec_h0) {
+      if (kCorr && spec_h0 && corr_h0) {
+        h0_corr<GEN, kCacheIv>(ds, T, GB, c, iv, u, gm, kappa, alpha, st, exact, Hz);
+      } else if (spec_h0) {
         h0_spec<GEN>(ds, T, kappa, alpha);
       } else {  // liblbfgs: H0 = (s.y / y.y) I
         const float hscale = (nh > 0) ? ys_last * rcpf(yy_last) : rsqrtf(gg);
@@ -1419,7 +1505,7 @@ static SolveKernelFn solve_kernel_for(int m, bool general, bool pad, bool f16) {
 }
 
 static size_t solve_lds_bytes(int m, int wpg) {
-  return (size_t)wpg * m * 2 * 64 * sizeof(float4);
+  return (size_t)wpg * (2 * m + 1) * 64 * sizeof(float4);
 }
 
 // Waves of this instantiation one CU holds at once with workgroups of `wpg`

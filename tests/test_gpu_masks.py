@@ -5,10 +5,14 @@ The three raised-cosine filters of genfilter (python3-6/pyNCS/denoisetools.py:
 call it), the PSF-derived OTF of genpsfparam (denoisetools.py:103-115) and
 the circular-pupil OTF mask of the benchmark, each on a synthetic filament
 frame through reduceNoise: every tile's final cost within 1e-5 relative of
-the FP64 oracle's, and the corrected image within 1e-3 of the frame's
-largest value of the oracle's (the stop rule |g| <= 1e-5 max(1, |u|) leaves
-bright pixels, whose likelihood curvature is small, that much room).  Also
-the FP16-split precision on the same inputs.
+the FP64 oracle's.  Pixels: the stop rule |g| <= 1e-5 max(1, |u|) leaves
+bright pixels, whose likelihood curvature is small, room around the
+minimiser (up to 2e-3 of the frame's largest value for these masks, for the
+oracle as for the GPU), so two solvers that both stop by it can differ by
+about that much.  The corrected image is therefore measured against the
+minimiser itself, the FP64 oracle run to |g| <= 1e-11 max(1, |u|), and must
+lie no further from it than 1.5 times the oracle's own stop point under the
+default rule.  Also the FP16-split precision on the same inputs.
 """
 import numpy
 import pytest
@@ -36,13 +40,19 @@ def test_generated_masks_match_the_oracle(gpu_ctx, kind, precision):
     fr = sim.make_frame(300 + len(kind), (128, 160))
     out, info = ncs.reduceNoise([fr["data"]], fr["gamma"], mask, 0.1, return_info=True,
                                 params=_lib.default_params(precision=precision))
-    ref, rinfo = oracle.reduce_noise(fr["data"].astype(numpy.float64),
-                                     fr["gamma"].astype(numpy.float64),
-                                     masks.to_fft_order(mask), 0.1)
+    d64, g64 = fr["data"].astype(numpy.float64), fr["gamma"].astype(numpy.float64)
+    ref, rinfo = oracle.reduce_noise(d64, g64, masks.to_fft_order(mask), 0.1)
+    tight, _ = oracle.reduce_noise(d64, g64, masks.to_fft_order(mask), 0.1,
+                                   param=oracle.default_param(epsilon=1e-11,
+                                                              max_iterations=2000))
     rel = numpy.abs(info["cost"].astype(numpy.float64) - rinfo["cost"]) / numpy.abs(rinfo["cost"])
     diff = float(numpy.max(numpy.abs(out[0] - ref)))
-    print("  tiles %d, status %s, cost max rel %.2e, pixel max |d| %.2e"
-          % (len(rel), numpy.unique(info["status"]).tolist(), rel.max(), diff), flush=True)
+    err = float(numpy.max(numpy.abs(out[0] - tight)))
+    err_ref = float(numpy.max(numpy.abs(ref - tight)))
+    print("  tiles %d, status %s, cost max rel %.2e, pixel max |d| %.2e vs the oracle; "
+          "from the minimiser %.2e (oracle's stop point %.2e)"
+          % (len(rel), numpy.unique(info["status"]).tolist(), rel.max(), diff, err, err_ref),
+          flush=True)
     assert numpy.all(info["status"] == 0)
     assert rel.max() <= 1e-5
-    assert diff <= 1e-3 * float(numpy.max(numpy.abs(ref)))
+    assert err <= 1.5 * err_ref

@@ -48,7 +48,7 @@ def test_default_params():
     assert p.max_iterations == 200 and p.history == 5
     assert numpy.isclose(p.epsilon, 1e-5) and p.linesearch == _lib.LS_MORE_THUENTE
     assert numpy.isclose(p.ftol, 1e-4) and numpy.isclose(p.gtol, 0.9)
-    assert p.precond == 1 and _lib.liblbfgs_params().precond == 0
+    assert p.precond == 2 and _lib.liblbfgs_params().precond == 0
     assert p.precision == _lib.PRECISION_FP32  # FP32 basis changes unless asked
     k = _lib.kernel_semantics_params()
     assert k.history == 8 and k.linesearch == _lib.LS_ARMIJO and numpy.isclose(k.min_step, 1e-6)
