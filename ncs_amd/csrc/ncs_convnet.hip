@@ -493,7 +493,9 @@ void net_fwd_kernel(const float *__restrict__ xin, float *__restrict__ yout,
   extern __shared__ __align__(16) unsigned char lds_raw[];
   i4 *pk = reinterpret_cast<i4 *>(lds_raw);
   float *bl = reinterpret_cast<float *>(pk + kFwdPacks * 64);
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // the wave index is uniform: read it into a scalar register so the tile
+  // loop's index arithmetic and frame-table reads stay on the scalar unit
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   float *io = bl + 128 + wave * kNetWaveFloats;
   const int col = lane & 15, grp = lane >> 4;
   stage_forward(packs, P, pk, bl);
@@ -509,6 +511,8 @@ void net_fwd_kernel(const float *__restrict__ xin, float *__restrict__ yout,
   }
 }
 
+typedef __attribute__((address_space(4))) const FrameDev FrameConstN;
+
 // Inference mode over frames: gather each window exactly like the solver,
 // normalise, run the net and write u0 = d + sigma * y as [tiles][256].
 __global__ __launch_bounds__(64 * kNetWaves) __attribute__((amdgpu_waves_per_eu(2)))
@@ -518,7 +522,12 @@ void net_warm_kernel(const FrameDev *__restrict__ frames, int n_frames, long lon
   extern __shared__ __align__(16) unsigned char lds_raw[];
   i4 *pk = reinterpret_cast<i4 *>(lds_raw);
   float *bl = reinterpret_cast<float *>(pk + kFwdPacks * 64);
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // the wave index is uniform: read it into a scalar register so the tile
+  // loop's index arithmetic and frame-table reads stay on the scalar unit
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  // the frame table through the constant address space (scalar loads; the
+  // kernel never writes it)
+  const FrameConstN *fc = (const FrameConstN *)frames;
   float *io = bl + 128 + wave * kNetWaveFloats;
   float *dkeep = io + kNetIoFloats;
   const int col = lane & 15, grp = lane >> 4;
@@ -528,11 +537,11 @@ void net_warm_kernel(const FrameDev *__restrict__ frames, int n_frames, long lon
     int lo = 0, hi = n_frames - 1;
     while (lo < hi) {
       int mid = (lo + hi + 1) >> 1;
-      if (frames[mid].first <= t) lo = mid; else hi = mid - 1;
+      if (fc[mid].first <= t) lo = mid; else hi = mid - 1;
     }
-    const FrameDev F = frames[lo];
-    const long long local = t - F.first;
-    const int a = (int)(local / F.ty), b = (int)(local % F.ty);
+    const FrameConstN &F = fc[lo];
+    const int local = (int)(t - F.first);  // a frame has fewer than 2^31 windows
+    const int a = local / F.ty, b = local - a * F.ty;
     int bx, by, hv = 16, wv = 16;
     if (F.geom == NCS_GEOM_REFERENCE) {
       bx = min(-1 + 14 * a, F.H - 15);
@@ -547,14 +556,16 @@ void net_warm_kernel(const FrameDev *__restrict__ frames, int n_frames, long lon
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       const int row = 4 * grp + r;
+      // 32-bit element offsets from the uniform base pointers (the host keeps
+      // every frame below 2^31 elements)
       if (F.geom == NCS_GEOM_REFERENCE) {
         const int gr = min(max(bx + row, 0), F.H - 1), gc = min(max(by + col, 0), F.W - 1);
-        d[r] = F.data[(long long)gr * F.ds + gc];
-        gm[r] = F.gamma[(long long)gr * F.gs + gc];
+        d[r] = F.data[(unsigned int)(gr * F.ds + gc)];
+        gm[r] = F.gamma[(unsigned int)(gr * F.gs + gc)];
       } else {
         const bool ok = row < hv && col < wv;
-        d[r] = ok ? F.data[(long long)(bx + row) * F.ds + by + col] : 0.0f;
-        gm[r] = ok ? F.gamma[(long long)(bx + row) * F.gs + by + col] : 0.0f;
+        d[r] = ok ? F.data[(unsigned int)((bx + row) * F.ds + by + col)] : 0.0f;
+        gm[r] = ok ? F.gamma[(unsigned int)((bx + row) * F.gs + by + col)] : 0.0f;
       }
     }
     float sigma;
