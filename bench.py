@@ -964,6 +964,9 @@ def main():
               "during the solve, and this build runs and reports a single MI355X only "
               "(--gpus %d requested)." % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.history is not None and not (args.schedule_cache and args.only):
+        ap.error("--history applies to profiler runs of one path from a cached schedule: "
+                 "give --only and --schedule-cache with it")
 
     if args.workload == "uneven":
         if args.uneven_history is not None:

@@ -13,3 +13,13 @@ def test_more_than_one_gpu_is_refused():
     assert p.returncode == 2
     assert "one GPU" in p.stderr or "single MI355X" in p.stderr
     assert p.stdout.strip() == ""
+
+
+def test_history_override_needs_a_single_cached_path():
+    """--history rewrites the cached schedule of a profiler run; without
+    --only and --schedule-cache it would have nothing to act on."""
+    p = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--history", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert p.returncode == 2
+    assert "--history" in p.stderr
+    assert p.stdout.strip() == ""
