@@ -14,8 +14,8 @@ import bench  # noqa: E402
 
 # (kernel, path, precision) of the bench line's roofline entries: the
 # headline and the paths it is chosen from, the FP16-split key, and the
-# uneven batch's linear start, each at both histories (1 and 2) the
-# autotuner picks from run to run
+# uneven batch's linear start, each at the histories (1, 2 and 3) the
+# autotuner chooses between in practice
 NEEDED = [
     ("solve_kernel<1,false,false,false>", "linear", "fp32"),
     ("solve_kernel<2,false,false,false>", "linear", "fp32"),
@@ -24,8 +24,12 @@ NEEDED = [
     ("solve_kernel<5,false,false,false>", "plain", "fp32"),
     ("solve_kernel<1,false,false,true>", "linear", "f16split"),
     ("solve_kernel<2,false,false,true>", "linear", "f16split"),
+    ("solve_kernel<3,false,false,false>", "linear", "fp32"),
+    ("solve_kernel<3,false,false,false>", "tuned", "fp32"),
+    ("solve_kernel<3,false,false,true>", "linear", "f16split"),
     ("solve_kernel<1,false,true,false>", "uneven_linear", "fp32"),
     ("solve_kernel<2,false,true,false>", "uneven_linear", "fp32"),
+    ("solve_kernel<3,false,true,false>", "uneven_linear", "fp32"),
 ]
 
 
