@@ -193,3 +193,38 @@ def test_stalled_fp16_search_retries_in_fp32(gpu_ctx):
          "max rel cost", float(rel.max()))
     assert numpy.all(info["status"] == 0)
     assert rel.max() <= 1e-5
+
+
+@pytest.mark.parametrize("precision", [FP32, F16])
+def test_failed_corrected_search_restarts_from_the_data(gpu_ctx, precision):
+    """A line search that fails under the curvature-corrected initial inverse
+    Hessian restarts the tile from the plain start with the spectral one.  On
+    this bright scene under a random 12 x 12 mask (history 1, alpha 0.1) the
+    spectral H0 alone ends 2-3 tiles in failed searches and a retry from the
+    failing point 1; with the restart every tile converges.  Costs: within
+    1e-5 of the FP64 oracle where it converged, no higher where its
+    liblbfgs restatement stopped early (random masks make some of these
+    tiles stiff for it too)."""
+    from ncs_amd import ncs
+    r = 12
+    fr = sim.make_frame(101, (96, 96))
+    mask = masks.random_mask(r, numpy.random.default_rng(r))
+    try:
+        h = ncs.NCSHip(size=r, params=_lib.default_params(precision=precision, history=1))
+        h.setOTFMask(mask)
+        h.setGamma(fr["gamma"])
+        out, info = h.reduceNoise([fr["data"]], 0.1, return_info=True)
+    finally:
+        gpu_ctx.set_otf_mask(masks.to_fft_order(masks.otf_mask()).astype(numpy.float32))
+    ref, rinfo = oracle.reduce_noise(fr["data"].astype(numpy.float64),
+                                     fr["gamma"].astype(numpy.float64),
+                                     masks.to_fft_order(mask), 0.1)
+    rel = (info["cost"].astype(numpy.float64) - rinfo["cost"]) / numpy.abs(rinfo["cost"])
+    conv = numpy.isin(rinfo["status"], (0, 2))
+    _say("random 12x12 mask, history 1, precision", precision, "status",
+         dict(zip(*[x.tolist() for x in numpy.unique(info["status"], return_counts=True)])),
+         "oracle converged", int(conv.sum()), "of", conv.size,
+         "max rel cost", float(numpy.abs(rel[conv]).max()))
+    assert numpy.all(info["status"] == 0)
+    assert numpy.abs(rel[conv]).max() <= 1e-5
+    assert numpy.all(rel[~conv] <= 1e-5)
