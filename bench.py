@@ -55,6 +55,14 @@ N_FRAMES, H, W, ALPHA, HISTORY, MAX_IT = 10, 2048, 2048, 0.1, 5, 200
 CORR_H0 = "curvature-corrected (precond=2)"
 
 
+def linstart_slope():
+    """The linear start's committed slopes in ln(alpha) (None without the
+    file): frames at alpha 0.1, the batch's, use the coefficients unchanged;
+    the uneven batch's frames (alpha 0.05-0.3) use them at their own alpha."""
+    from ncs_amd import linstart
+    return linstart.load_slope()
+
+
 def _frame(seed):
     from ncs_amd import sim
     fr = sim.make_frame(seed, (H, W))
@@ -330,7 +338,7 @@ def run_gpu(frames_np, args, weights, lin_coef=None):
     if lin_coef is not None:
         # learned linear start, fused into the solve kernel, with a schedule
         # tuned for it (its tiles take fewer iterations)
-        ctx.set_linear_start(lin_coef)
+        ctx.set_linear_start(lin_coef, linstart_slope())
         if sched is None or "linear" not in sched:
             tune_l = ctx.autotune(desc, _lib.default_params(max_iterations=MAX_IT,
                                                             start=_lib.START_LINEAR),
@@ -556,7 +564,7 @@ def run_uneven_path(args, frames, history):
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
-    ctx.set_linear_start(linstart.load())
+    ctx.set_linear_start(linstart.load(), linstart_slope())
     iters = torch.zeros(total, dtype=torch.int32, device="cuda:0")
     status = torch.zeros_like(iters)
     cost = torch.zeros(total, dtype=torch.float32, device="cuda:0")
@@ -637,7 +645,7 @@ def run_uneven(args, frames):
     if not args.no_linear:
         from ncs_amd import linstart
         if os.path.exists(linstart.DEFAULT_COEF):
-            ctx.set_linear_start(linstart.load())
+            ctx.set_linear_start(linstart.load(), linstart_slope())
             pl_base = _lib.default_params(max_iterations=MAX_IT, start=_lib.START_LINEAR)
             tune_l = ctx.autotune(desc, pl_base, tune_history=True)
             pl_p = _lib.default_params(max_iterations=MAX_IT, start=_lib.START_LINEAR,
@@ -649,6 +657,8 @@ def run_uneven(args, frames):
             lin = {"value": round(pix / lwall / 1e6, 2), "unit": "Mpix/s",
                    "ms_per_step": round(lwall * 1e3, 3), "kernel_ms": round(lkms, 3),
                    "autotune": tune_l, "iterations_mean": round(float(lit.mean()), 3),
+                   "coefficients": ("alpha 0.1 fit, slopes in ln(alpha) for each frame's alpha"
+                                    if linstart_slope() is not None else "alpha 0.1 fit"),
                    "cost_vs_plain_max_rel": float(lrel.max()),
                    "matched_1e5": bool(lrel.max() <= 1e-5),
                    "status": {str(k): int(v) for k, v in zip(*numpy.unique(lst, return_counts=True))}}
@@ -756,7 +766,7 @@ def run_stream(args, weights, lin_coef=None):
     results = {}
     modes = [("plain", _lib.START_PLAIN)]
     if lin_coef is not None:
-        ctx.set_linear_start(lin_coef)
+        ctx.set_linear_start(lin_coef, linstart_slope())
         modes.append(("linear", _lib.START_LINEAR))
     if weights is not None:
         from ncs_amd import warmstart

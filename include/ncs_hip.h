@@ -313,8 +313,17 @@ int ncs_hip_net_backward(ncs_ctx *ctx, const float *params, const float *x,
  * tile whenever it scores the lower cost (params->fallback). */
 #define NCS_LINEAR_START_PARAMS 513
 
-/* coef: A[16][16], then B[16][16] (indices [a][b] of Z), then c. */
+/* coef: A[16][16], then B[16][16] (indices [a][b] of Z), then c.  Clears
+ * any slopes set by ncs_hip_set_linear_start_alpha. */
 int ncs_hip_set_linear_start(ncs_ctx *ctx, const float *coef);
+
+/* No reference counterpart.  Slopes of the linear start's coefficients in
+ * ln(alpha), same layout as coef (NCS_LINEAR_START_PARAMS floats): a frame
+ * at alpha uses coef + ln(alpha / alpha_ref) * slope, so one fitted set
+ * serves a batch whose frames carry different alphas.  Frames at alpha_ref
+ * use coef unchanged.  Call after ncs_hip_set_linear_start (NCS_E_NOWEIGHTS
+ * otherwise); alpha_ref > 0. */
+int ncs_hip_set_linear_start_alpha(ncs_ctx *ctx, const float *slope, float alpha_ref);
 
 /* ---- autotuner (no reference counterpart) ---- */
 

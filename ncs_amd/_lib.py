@@ -127,6 +127,7 @@ SIGNATURES = {
     "ncs_hip_eval_tiles": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _i, _f]),
     "ncs_hip_net_set_weights": (_i, [_vp, _vp]),
     "ncs_hip_set_linear_start": (_i, [_vp, _vp]),
+    "ncs_hip_set_linear_start_alpha": (_i, [_vp, _vp, ctypes.c_float]),
     "ncs_hip_net_predict_frames_device": (_i, [_vp, _F, _i, _vp, _vp]),
     "ncs_hip_net_forward": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _vp]),
     "ncs_hip_net_backward": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp]),
@@ -279,13 +280,23 @@ class Context(object):
         self.mask_fft = m.copy()
         self.window = r
 
-    def set_linear_start(self, coef):
-        """Coefficients [A (16x16), B (16x16), c] of the learned linear start."""
+    def set_linear_start(self, coef, slope=None, alpha_ref=0.1):
+        """Coefficients [A (16x16), B (16x16), c] of the learned linear start,
+        fitted at alpha_ref, and optionally their slopes in ln(alpha) (same
+        layout): a frame at alpha then uses coef + ln(alpha / alpha_ref) *
+        slope.  Without slopes the coefficients are used at every alpha."""
         c = numpy.ascontiguousarray(coef, dtype=numpy.float32)
         if c.shape != (LINEAR_START_PARAMS,):
             raise ValueError("expected %d coefficients" % LINEAR_START_PARAMS)
         check(self.lib.ncs_hip_set_linear_start(self.handle, _ptr(c)),
               "ncs_hip_set_linear_start")
+        if slope is not None:
+            sl = numpy.ascontiguousarray(slope, dtype=numpy.float32)
+            if sl.shape != (LINEAR_START_PARAMS,):
+                raise ValueError("expected %d slopes" % LINEAR_START_PARAMS)
+            check(self.lib.ncs_hip_set_linear_start_alpha(self.handle, _ptr(sl),
+                                                          float(alpha_ref)),
+                  "ncs_hip_set_linear_start_alpha")
 
     def reduce_frames(self, datas, gammas, outs, alphas, geometry, params):
         """Host-memory batch; returns per-tile (iterations, status, cost)."""

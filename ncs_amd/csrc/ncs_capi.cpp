@@ -117,6 +117,8 @@ struct ncs_ctx {
   std::vector<float> tab_host = std::vector<float>(kTableFloats, 0.0f);
   bool has_linear = false;
   float lin_c = 0.0f;
+  float lin_c1 = 0.0f, lin_aref = 0.1f;  // the linear start's alpha slope
+  bool has_slope = false;
   float k_norm = 0.0f;  // max row sum |2 q_ee| + |2 q_oo| of the mask tables
   // autotuned schedules, keyed by (general mask, padded geometry, FP16 split)
   std::map<TuneKey, Schedule> tuned;
@@ -382,6 +384,8 @@ extern "C" int ncs_hip_set_linear_start(ncs_ctx *ctx, const float *coef) {
       const int k = j * 16 + 4 * g + r;
       tab[lane * kTableLaneFloats + 16 + r] = coef[k];
       tab[lane * kTableLaneFloats + 20 + r] = coef[256 + k];
+      tab[lane * kTableLaneFloats + 4 * kTableSlope + r] = 0.0f;  // slopes cleared
+      tab[lane * kTableLaneFloats + 4 * kTableSlope + 4 + r] = 0.0f;
     }
   }
   HIPCHK(set_device(ctx));
@@ -389,7 +393,36 @@ extern "C" int ncs_hip_set_linear_start(ncs_ctx *ctx, const float *coef) {
   HIPCHK(hipMemcpy(ctx->tables.p, tab.data(), tab.size() * sizeof(float),
                    hipMemcpyHostToDevice));
   ctx->lin_c = coef[512];
+  ctx->lin_c1 = 0.0f;
+  ctx->has_slope = false;
   ctx->has_linear = true;
+  return NCS_OK;
+}
+
+extern "C" int ncs_hip_set_linear_start_alpha(ncs_ctx *ctx, const float *slope,
+                                             float alpha_ref) {
+  if (!ctx || !slope) return fail(NCS_E_ARG, "bad arguments");
+  if (!ctx->has_linear) return fail(NCS_E_NOWEIGHTS, "set the linear start's coefficients first");
+  if (!(alpha_ref > 0.0f) || !std::isfinite(alpha_ref))
+    return fail(NCS_E_ARG, "alpha_ref must be positive and finite");
+  for (int i = 0; i < NCS_LINEAR_START_PARAMS; i++)
+    if (!std::isfinite(slope[i])) return fail(NCS_E_ARG, "non-finite slope");
+  std::vector<float> &tab = ctx->tab_host;
+  for (int lane = 0; lane < 64; lane++) {
+    const int g = lane >> 4, j = lane & 15;
+    for (int r = 0; r < 4; r++) {
+      const int k = j * 16 + 4 * g + r;
+      tab[lane * kTableLaneFloats + 4 * kTableSlope + r] = slope[k];
+      tab[lane * kTableLaneFloats + 4 * kTableSlope + 4 + r] = slope[256 + k];
+    }
+  }
+  HIPCHK(set_device(ctx));
+  HIPCHK(ctx->tables.ensure(tab.size() * sizeof(float)));
+  HIPCHK(hipMemcpy(ctx->tables.p, tab.data(), tab.size() * sizeof(float),
+                   hipMemcpyHostToDevice));
+  ctx->lin_c1 = slope[512];
+  ctx->lin_aref = alpha_ref;
+  ctx->has_slope = true;
   return NCS_OK;
 }
 
@@ -595,6 +628,9 @@ static SolveArgs make_args(const ncs_ctx *ctx, const FrameDev *d_frames, int n_f
   a.precond = p->precond;
   a.lin_start = p->start == NCS_START_LINEAR && u0 == nullptr;
   a.lin_c = ctx->lin_c;
+  a.lin_c1 = ctx->lin_c1;
+  a.lin_aref = ctx->lin_aref;
+  a.lin_slope = ctx->has_slope ? 1 : 0;
   a.k_norm = ctx->k_norm;
   return a;
 }

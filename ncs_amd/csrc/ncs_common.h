@@ -27,13 +27,16 @@ constexpr float kFitMin = 1.0e-6f; // negative-value guard (ncs.cl:45, ncs.c:169
 
 // Per-lane tables, kTableF4 16-byte entries per lane: the real-DFT basis
 // entries in FP32 (two), the mask's even-even and odd-odd weights, the
-// learned linear start's two coefficient sets, and the basis entries again
-// as FP16 hi/lo pairs for the FP16 matrix-core basis changes (four; see
-// ncs_solver.hip, LdsTables).  The solver keeps entries kSolveTableFirst..
-// in LDS.
-constexpr int kTableF4 = 10;
+// learned linear start's two coefficient sets, the basis entries again as
+// FP16 hi/lo pairs for the FP16 matrix-core basis changes (four; see
+// ncs_solver.hip, LdsTables), and the linear start's two slopes in
+// ln(alpha) (kTableSlope, kTableSlope + 1).  The solver keeps entries
+// kSolveTableFirst .. kSolveTableFirst + kSolveTableF4 - 1 in LDS; the
+// slopes, read once per tile, stay in the global table.
+constexpr int kTableF4 = 12;
 constexpr int kSolveTableFirst = 2;
-constexpr int kSolveTableF4 = kTableF4 - kSolveTableFirst;
+constexpr int kSolveTableF4 = 8;
+constexpr int kTableSlope = 10;
 constexpr int kTableLaneFloats = 4 * kTableF4;
 constexpr int kTableFloats = kWave * kTableLaneFloats;
 
@@ -74,6 +77,9 @@ struct SolveArgs {
   int precond;                 // spectral initial inverse Hessian (else s.y / y.y)
   int lin_start;               // learned linear start (coefficients in the tables)
   float lin_c;                 // its DC bias
+  float lin_c1;                // the bias's slope in ln(alpha)
+  float lin_aref;              // alpha the coefficients were fitted at
+  int lin_slope;               // nonzero: the coefficients follow alpha
   float k_norm;                // bound on the NC Hessian's norm (row sums of the mask tables)
 };
 

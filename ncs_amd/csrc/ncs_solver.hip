@@ -765,9 +765,10 @@ __device__ __forceinline__ void ll_trial(float t, const float (&u)[4], const flo
 }
 
 // One to kMaxWavesPerGroup waves per workgroup, each solving its own tiles;
-// a wave's L-BFGS history lives in its slice of LDS as two rings of M vectors
-// (each lane keeps its own 4 values at float4 index slot * 64 + lane, so no
-// lane reads another lane's LDS and no barrier is needed).  PAD enables
+// a wave's L-BFGS history lives in its slice of LDS as two rings of M vectors,
+// followed by one vector of scratch for the curvature-corrected H0 (each
+// lane keeps its own 4 values at float4 index slot * 64 + lane, so no lane
+// reads another lane's LDS and no barrier is needed).  PAD enables
 // zero-padded pixels (block geometry).
 //
 // Occupancy target: 5 waves per SIMD (at most 96 VGPRs) for histories up to
@@ -984,12 +985,31 @@ void solve_kernel(SolveArgs A) {
           const float mgam = lane0(zg[0]) * (1.0f / 16.0f);
           const float sig = sqrtf(fmaxf(mu, 1.0f) + mgam);
           const float isig = rcpf(sig);
-          const f4 la = T.la(), lb = T.lb();
+          f4 la = T.la(), lb = T.lb();
+          float lc = A.lin_c;
+          if (A.lin_slope && alpha != A.lin_aref) {
+            // the coefficients at this frame's alpha: plus ln(alpha /
+            // alpha_ref) times their slopes (global table, read once per
+            // tile); frames at alpha_ref keep the coefficients exactly
+            const float ell = __builtin_bit_cast(
+                float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(
+                           int, (__builtin_amdgcn_logf(alpha) - __builtin_amdgcn_logf(A.lin_aref)) *
+                                    kLn2)));
+            const f4 *sl = reinterpret_cast<const f4 *>(A.tables + lane * kTableLaneFloats) +
+                           kTableSlope;
+            const f4 la1 = sl[0], lb1 = sl[1];
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+              la[r] = fmaf(ell, la1[r], la[r]);
+              lb[r] = fmaf(ell, lb1[r], lb[r]);
+            }
+            lc = fmaf(ell, A.lin_c1, lc);
+          }
 #pragma unroll
           for (int r = 0; r < 4; r++)
             pz[r] = sig * fmaf(la[r], zs[r] * isig, lb[r] * zg[r] * (isig * isig));
           if (lane == 0)  // DC: x0 has none; the bias term lives here
-            pz[0] = sig * fmaf(lb[0], zg[0] * (isig * isig), 16.0f * A.lin_c);
+            pz[0] = sig * fmaf(lb[0], zg[0] * (isig * isig), 16.0f * lc);
           to_pix16<false>(pz, T, GB, st, wp, exact);
 #pragma unroll
           for (int r = 0; r < 4; r++) {

@@ -29,6 +29,8 @@ from . import _lib
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_COEF = os.path.join(HERE, "weights", "linstart_otf_a0.1.npy")
+DEFAULT_SLOPE = os.path.join(HERE, "weights", "linstart_otf_slope.npy")
+ALPHA_REF = 0.1  # the alpha DEFAULT_COEF was fitted at
 N_COEF = 2 * 256 + 1
 
 
@@ -96,8 +98,20 @@ def fit(d, g, u, ridge=1e-9):
     return numpy.concatenate([A.ravel(), B.ravel(), [c]]).astype(numpy.float32)
 
 
-def predict(d, g, coef):
-    """The start u0 of tiles [n, 16, 16] (float64), as the solver computes it."""
+def at_alpha(coef, slope, alpha, alpha_ref=ALPHA_REF):
+    """The coefficients a frame at alpha uses: coef + ln(alpha / alpha_ref) *
+    slope (coef itself without slopes or at alpha_ref)."""
+    coef = numpy.asarray(coef, numpy.float64)
+    if slope is None or alpha == alpha_ref:
+        return coef
+    return coef + numpy.log(alpha / alpha_ref) * numpy.asarray(slope, numpy.float64)
+
+
+def predict(d, g, coef, alpha=None, slope=None, alpha_ref=ALPHA_REF):
+    """The start u0 of tiles [n, 16, 16] (float64), as the solver computes it;
+    with slope and alpha, at that alpha's coefficients."""
+    if slope is not None and alpha is not None:
+        coef = at_alpha(coef, slope, alpha, alpha_ref)
     coef = numpy.asarray(coef, numpy.float64)
     A = coef[:256].reshape(16, 16)
     B = coef[256:512].reshape(16, 16)
@@ -105,6 +119,52 @@ def predict(d, g, coef):
     X0, X1, sig = features(d, g)
     one = spec(numpy.ones((16, 16)))
     return numpy.asarray(d, numpy.float64) + sig * pix(A * X0 + B * X1 + c * one)
+
+
+def fit_slope(d, g, solutions, coef, alpha_ref=ALPHA_REF, ridge=1e-9):
+    """Slopes in ln(alpha) of the coefficients, given the alpha_ref fit: for
+    each coefficient, the least squares of the alpha_ref start's residual
+    Z((u*_alpha - d) / sigma) - (A Z(x0) + B Z(x1) + c Z(1)) on
+    ln(alpha / alpha_ref) [Z(x0), Z(x1), Z(1)] over every alpha of
+    solutions, a list of (alpha, u* [n, 16, 16]) for the same tiles.
+    Returns N_COEF float32 slopes [A', B', c'] in coef's layout."""
+    coef = numpy.asarray(coef, numpy.float64)
+    A0 = coef[:256].reshape(16, 16)
+    B0 = coef[256:512].reshape(16, 16)
+    c0 = coef[512]
+    X0, X1, sig = features(d, g)
+    one = spec(numpy.ones((16, 16)))
+    d64 = numpy.asarray(d, numpy.float64)
+    rows = []
+    for alpha, u in solutions:
+        Y = spec((numpy.asarray(u, numpy.float64) - d64) / sig)
+        rows.append((numpy.log(alpha / alpha_ref), Y - (A0 * X0 + B0 * X1 + c0 * one)))
+    A1 = numpy.zeros((16, 16))
+    B1 = numpy.zeros((16, 16))
+    c1 = 0.0
+    for a in range(16):
+        for b in range(16):
+            cols, ys = [], []
+            for ell, R in rows:
+                c = [ell * X0[:, a, b], ell * X1[:, a, b]]
+                if abs(one[a, b]) > 0.5:
+                    c.append(numpy.full(len(X0), ell * one[a, b]))
+                cols.append(numpy.stack(c, 1))
+                ys.append(R[:, a, b])
+            M = numpy.concatenate(cols)
+            G = M.T @ M + ridge * numpy.eye(M.shape[1]) * len(M)
+            w = numpy.linalg.solve(G, M.T @ numpy.concatenate(ys))
+            A1[a, b], B1[a, b] = w[0], w[1]
+            if len(w) == 3:
+                c1 = w[2]
+    return numpy.concatenate([A1.ravel(), B1.ravel(), [c1]]).astype(numpy.float32)
+
+
+def load_slope(path=DEFAULT_SLOPE):
+    """The committed slopes, or None when the file is absent."""
+    if not os.path.exists(path):
+        return None
+    return load(path)
 
 
 def save(coef, path=DEFAULT_COEF):
@@ -119,9 +179,10 @@ def load(path=DEFAULT_COEF):
     return c
 
 
-def set_context_coef(coef, device=0):
-    """Hand the coefficients to the library (NCS_START_LINEAR)."""
-    _lib.context(device).set_linear_start(coef)
+def set_context_coef(coef, device=0, slope=None, alpha_ref=ALPHA_REF):
+    """Hand the coefficients (and their alpha slopes) to the library
+    (NCS_START_LINEAR)."""
+    _lib.context(device).set_linear_start(coef, slope, alpha_ref)
 
 
 def main(argv=None):
@@ -132,7 +193,16 @@ def main(argv=None):
     f.add_argument("--alpha", type=float, default=0.1)
     f.add_argument("--workers", type=int, default=min(16, os.cpu_count() or 1))
     f.add_argument("--out", default=DEFAULT_COEF)
+    fs = sub.add_parser("fit-slope", help="slopes in ln(alpha) for a fitted set")
+    fs.add_argument("--tiles", type=int, default=60_000)
+    fs.add_argument("--alphas", default="0.05,0.2,0.3")
+    fs.add_argument("--coef", default=DEFAULT_COEF)
+    fs.add_argument("--alpha-ref", type=float, default=ALPHA_REF)
+    fs.add_argument("--workers", type=int, default=min(16, os.cpu_count() or 1))
+    fs.add_argument("--out", default=DEFAULT_SLOPE)
     args = ap.parse_args(argv)
+    if args.cmd == "fit-slope":
+        return _fit_slope_main(args)
     from . import warmstart
     log = lambda *a: print(*a, flush=True)
     d, g = warmstart.gen_windows(args.tiles, workers=args.workers, log=log)
@@ -157,6 +227,41 @@ def main(argv=None):
                       "held_out_cost_max_rel": float(rel.max()),
                       "status_linear": {str(k): int(v) for k, v in
                                         zip(*numpy.unique(st_l, return_counts=True))}}))
+    return 0
+
+
+def _fit_slope_main(args):
+    """Solutions of the same training windows at each alpha on the GPU, the
+    slopes fitted to them, and held-out iterations at each alpha (history 1,
+    the default initial Hessian) with the alpha_ref coefficients alone and
+    with the slopes."""
+    from . import warmstart
+    log = lambda *a: print(*a, flush=True)
+    alphas = [float(a) for a in args.alphas.split(",")]
+    coef = load(args.coef)
+    d, g = warmstart.gen_windows(args.tiles, workers=args.workers, log=log)
+    de, ge = warmstart.gen_windows(4096, seed0=20_000_000, workers=args.workers, log=log)
+    t0 = time.time()
+    sols = []
+    for a in alphas:
+        _, _, tu = warmstart.solve_pairs(d, g, alpha=a, log=log)
+        sols.append((a, tu.cpu().numpy()))
+    slope = fit_slope(d, g, sols, coef, args.alpha_ref)
+    log("fit-slope: %d tiles x %d alphas in %.1f s" % (len(d), len(alphas), time.time() - t0))
+    save(slope, args.out)
+    ctx = _lib.context(0)
+    report = {"slope": args.out, "tiles": len(d), "alphas": alphas, "alpha_ref": args.alpha_ref}
+    for a in alphas + [args.alpha_ref]:
+        row = {}
+        for label, sl in (("fixed", None), ("adapted", slope)):
+            ctx.set_linear_start(coef, sl, args.alpha_ref)
+            _, it, st, c = ctx.solve_tiles(de, ge, a, _lib.default_params(
+                start=_lib.START_LINEAR, history=1))
+            row[label] = {"iterations": float(it.mean()),
+                          "status": {str(k): int(v) for k, v in
+                                     zip(*numpy.unique(st, return_counts=True))}}
+        report["held_out_alpha_%g" % a] = row
+    print(json.dumps(report))
     return 0
 
 

@@ -56,6 +56,44 @@ def test_kernel_start_matches_numpy_predictor(otf):
     assert numpy.array_equal(u0, d)
 
 
+def test_kernel_start_follows_alpha_with_slopes(otf):
+    """With slopes set, a frame at alpha starts from the predictor at
+    coef + ln(alpha / 0.1) * slope (FP64 numpy, 2e-5 of the tile scale); at
+    alpha 0.1 itself the start is bit-identical to the one without slopes.
+    Slopes need coefficients first."""
+    ctx = otf
+    d, g = _windows(82)
+    coef = _random_coef(3, scale=0.2)
+    slope = _random_coef(4, scale=0.1)
+    p = _lib.default_params(start=_lib.START_LINEAR, max_iterations=0, fallback=0)
+    ctx.set_linear_start(coef)
+    u_ref, _, _, _ = ctx.solve_tiles(d, g, 0.1, p)
+    ctx.set_linear_start(coef, slope, 0.1)
+    u_same, _, _, _ = ctx.solve_tiles(d, g, 0.1, p)
+    assert numpy.array_equal(u_same, u_ref)
+    for alpha in (0.05, 0.3):
+        u, it, _, _ = ctx.solve_tiles(d, g, alpha, p)
+        ref = linstart.predict(d, g, coef, alpha=alpha, slope=slope, alpha_ref=0.1)
+        err = numpy.abs(u - ref).max(axis=(1, 2))
+        scale = numpy.abs(ref).max(axis=(1, 2))
+        _say("alpha", alpha, "start vs numpy with slopes: max err / scale",
+             float((err / scale).max()))
+        assert numpy.all(it == 0)
+        assert numpy.all(err <= 2e-5 * scale)
+    # set_linear_start alone clears the slopes
+    ctx.set_linear_start(coef)
+    u_cleared, _, _, _ = ctx.solve_tiles(d, g, 0.3, p)
+    assert numpy.allclose(u_cleared, linstart.predict(d, g, coef), rtol=2e-5, atol=1e-3)
+    # slopes without coefficients: NCS_E_NOWEIGHTS (-5) on a fresh context
+    fresh = _lib.Context(0)
+    try:
+        rc = fresh.lib.ncs_hip_set_linear_start_alpha(
+            fresh.handle, slope.ctypes.data_as(_lib.ctypes.c_void_p), 0.1)
+        assert rc == -5
+    finally:
+        fresh.close()
+
+
 def test_linear_start_reaches_plain_costs_with_fewer_iterations(otf):
     ctx = otf
     if not os.path.exists(linstart.DEFAULT_COEF):

@@ -82,3 +82,53 @@ def test_stored_coefficients_load():
     c = linstart.load()
     assert c.shape == (linstart.N_COEF,) and c.dtype == numpy.float32
     assert numpy.all(numpy.isfinite(c))
+
+
+def test_fit_slope_recovers_slopes_built_in_pixel_space():
+    """Solutions whose gains move with ln(alpha / 0.1): a uniform gain
+    a(alpha) = 0.3 + 0.05 ln(alpha / 0.1) on the centred data, b(alpha) = 0.5
+    - 0.1 ln(alpha / 0.1) on gamma and a shift of (2 + 0.4 ln(alpha / 0.1))
+    sigma.  Given the alpha = 0.1 fit, fit_slope recovers the three slopes
+    (A' everywhere off DC, B', c')."""
+    d, g = _tiles(2000, 5)
+    mu = d.mean(axis=(1, 2), keepdims=True)
+    sig = numpy.sqrt(numpy.maximum(mu, 1.0) + g.mean(axis=(1, 2), keepdims=True))
+
+    def solution(alpha):
+        ell = numpy.log(alpha / 0.1)
+        return (d + (0.3 + 0.05 * ell) * (d - mu) + (0.5 - 0.1 * ell) * g / sig
+                + (2.0 + 0.4 * ell) * sig)
+
+    coef = linstart.fit(d, g, solution(0.1))
+    slope = linstart.fit_slope(d, g, [(a, solution(a)) for a in (0.05, 0.2, 0.3)],
+                               coef).astype(numpy.float64)
+    A1 = slope[:256].reshape(16, 16)
+    B1 = slope[256:512].reshape(16, 16)
+    off_dc = numpy.ones((16, 16), bool)
+    off_dc[0, 0] = False
+    assert numpy.allclose(A1[off_dc], 0.05, atol=1e-5)
+    assert numpy.allclose(B1, -0.1, atol=1e-5)
+    assert slope[512] == pytest.approx(0.4, abs=1e-5)
+    # the predictor at each alpha then reproduces the solutions
+    for a in (0.05, 0.3):
+        u = linstart.predict(d, g, coef, alpha=a, slope=slope)
+        assert numpy.allclose(u, solution(a), rtol=1e-6, atol=1e-4)
+
+
+def test_predictor_at_the_reference_alpha_ignores_the_slopes():
+    d, g = _tiles(50, 6)
+    rng = numpy.random.default_rng(7)
+    coef = rng.uniform(-0.3, 0.3, linstart.N_COEF)
+    slope = rng.uniform(-0.1, 0.1, linstart.N_COEF)
+    u0 = linstart.predict(d, g, coef)
+    assert numpy.array_equal(linstart.predict(d, g, coef, alpha=linstart.ALPHA_REF,
+                                              slope=slope), u0)
+    assert not numpy.allclose(linstart.predict(d, g, coef, alpha=0.2, slope=slope), u0)
+
+
+def test_stored_slopes_load():
+    s = linstart.load_slope()
+    if s is None:
+        pytest.skip("no stored slopes")
+    assert s.shape == (linstart.N_COEF,) and s.dtype == numpy.float32
+    assert numpy.all(numpy.isfinite(s))
