@@ -513,17 +513,26 @@ def cpu_baseline(frames_np, budget_s):
 # uneven batch: mixed sizes and ROIs, zero-padded blocks, per-frame alpha
 # ---------------------------------------------------------------------------
 
-def make_uneven_frames():
+def make_uneven_frames(cache=None):
     """The uneven stack's frames, generated by a process pool (before the GPU
-    is initialised)."""
+    is initialised).  With `cache` (an .npz path) they are read from it when
+    present and written to it otherwise: profiler runs read the cache and
+    start no worker processes (the pool's shutdown signals its workers, and
+    under a counter-collecting profiler a signalled worker can hang)."""
     from ncs_amd.workloads import uneven_frame, uneven_specs
     shapes, alphas, seeds = uneven_specs()
+    if cache and os.path.exists(cache):
+        with numpy.load(cache) as z:
+            return [(z["d%d" % i], z["g%d" % i]) for i in range(len(shapes))]
     t0 = time.perf_counter()
     from multiprocessing import get_context
     with get_context("fork").Pool(host_threads()[0]) as pool:
         frames = pool.map(uneven_frame, list(zip(seeds, shapes)))
     print("[bench] uneven: generated %d frames in %.1f s" % (len(frames), time.perf_counter() - t0),
           file=sys.stderr, flush=True)
+    if cache:
+        numpy.savez(cache, **{k: v for i, (d, g) in enumerate(frames)
+                              for k, v in (("d%d" % i, d), ("g%d" % i, g))})
     return frames
 
 
@@ -955,6 +964,9 @@ def main():
     ap.add_argument("--history", type=int, default=None,
                     help="with --schedule-cache: run the tuned, linear and FP16-split paths at "
                          "this history instead of the cached one (profiler runs)")
+    ap.add_argument("--uneven-cache", default=None,
+                    help="uneven workload: .npz of its frames, read when present, written "
+                         "otherwise (profiler runs start no worker processes)")
     ap.add_argument("--uneven-history", type=int, default=None,
                     help="with --workload uneven: launch only the linear path at this "
                          "history and print its timing (profiler runs)")
@@ -980,10 +992,11 @@ def main():
 
     if args.workload == "uneven":
         if args.uneven_history is not None:
-            print(json.dumps(run_uneven_path(args, make_uneven_frames(), args.uneven_history)),
+            print(json.dumps(run_uneven_path(args, make_uneven_frames(args.uneven_cache),
+                                              args.uneven_history)),
                   flush=True)
         else:
-            print(json.dumps(run_uneven(args, make_uneven_frames())), flush=True)
+            print(json.dumps(run_uneven(args, make_uneven_frames(args.uneven_cache))), flush=True)
         return
     if args.workload == "stream":
         weights = None
