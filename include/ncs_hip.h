@@ -321,8 +321,9 @@ int ncs_hip_set_linear_start(ncs_ctx *ctx, const float *coef);
  * ln(alpha), same layout as coef (NCS_LINEAR_START_PARAMS floats): a frame
  * at alpha uses coef + ln(alpha / alpha_ref) * slope, so one fitted set
  * serves a batch whose frames carry different alphas.  Frames at alpha_ref
- * use coef unchanged.  Call after ncs_hip_set_linear_start (NCS_E_NOWEIGHTS
- * otherwise); alpha_ref > 0. */
+ * (and at alpha <= 0) use coef unchanged; ln(alpha / alpha_ref) is clamped
+ * to +-ln 8, so the slopes extrapolate at most 8x from alpha_ref.  Call
+ * after ncs_hip_set_linear_start (NCS_E_NOWEIGHTS otherwise); alpha_ref > 0. */
 int ncs_hip_set_linear_start_alpha(ncs_ctx *ctx, const float *slope, float alpha_ref);
 
 /* ---- autotuner (no reference counterpart) ---- */

@@ -426,6 +426,10 @@ constexpr float kH0Clamp = 0.5f;
 // tile from kappa at its start.
 constexpr float kH0CorrRatio = 30.0f;
 
+// The linear start's slopes in ln(alpha) are extrapolated at most a factor 8
+// from the alpha the coefficients were fitted at: ln 8.
+constexpr float kLinSlopeSpan = 2.0794415f;
+
 // x <- H0 x with the likelihood curvature's variation over the tile taken
 // to first order (precond 2).  The Hessian is P^-1 + D, P = (kappa I +
 // alpha K)^-1 (spectral, h0_spec) and D = diag(c_i - kappa) (pixels, c_i =
@@ -987,14 +991,16 @@ void solve_kernel(SolveArgs A) {
           const float isig = rcpf(sig);
           f4 la = T.la(), lb = T.lb();
           float lc = A.lin_c;
-          if (A.lin_slope && alpha != A.lin_aref) {
+          if (A.lin_slope && alpha != A.lin_aref && alpha > 0.0f) {
             // the coefficients at this frame's alpha: plus ln(alpha /
-            // alpha_ref) times their slopes (global table, read once per
-            // tile); frames at alpha_ref keep the coefficients exactly
+            // alpha_ref), clamped to +-kLinSlopeSpan, times their slopes
+            // (global table, read once per tile); frames at alpha_ref keep
+            // the coefficients exactly
+            const float ell0 =
+                (__builtin_amdgcn_logf(alpha) - __builtin_amdgcn_logf(A.lin_aref)) * kLn2;
             const float ell = __builtin_bit_cast(
                 float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(
-                           int, (__builtin_amdgcn_logf(alpha) - __builtin_amdgcn_logf(A.lin_aref)) *
-                                    kLn2)));
+                           int, fminf(fmaxf(ell0, -kLinSlopeSpan), kLinSlopeSpan))));
             const f4 *sl = reinterpret_cast<const f4 *>(A.tables + lane * kTableLaneFloats) +
                            kTableSlope;
             const f4 la1 = sl[0], lb1 = sl[1];

@@ -31,6 +31,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_COEF = os.path.join(HERE, "weights", "linstart_otf_a0.1.npy")
 DEFAULT_SLOPE = os.path.join(HERE, "weights", "linstart_otf_slope.npy")
 ALPHA_REF = 0.1  # the alpha DEFAULT_COEF was fitted at
+SLOPE_SPAN = numpy.log(8.0)  # slopes extrapolate at most 8x from alpha_ref
 N_COEF = 2 * 256 + 1
 
 
@@ -99,12 +100,14 @@ def fit(d, g, u, ridge=1e-9):
 
 
 def at_alpha(coef, slope, alpha, alpha_ref=ALPHA_REF):
-    """The coefficients a frame at alpha uses: coef + ln(alpha / alpha_ref) *
-    slope (coef itself without slopes or at alpha_ref)."""
+    """The coefficients a frame at alpha uses, as the solver forms them:
+    coef + ell * slope with ell = ln(alpha / alpha_ref) clamped to
+    +-SLOPE_SPAN; coef itself without slopes, at alpha_ref or for alpha <= 0."""
     coef = numpy.asarray(coef, numpy.float64)
-    if slope is None or alpha == alpha_ref:
+    if slope is None or alpha == alpha_ref or not alpha > 0:
         return coef
-    return coef + numpy.log(alpha / alpha_ref) * numpy.asarray(slope, numpy.float64)
+    ell = float(numpy.clip(numpy.log(alpha / alpha_ref), -SLOPE_SPAN, SLOPE_SPAN))
+    return coef + ell * numpy.asarray(slope, numpy.float64)
 
 
 def predict(d, g, coef, alpha=None, slope=None, alpha_ref=ALPHA_REF):

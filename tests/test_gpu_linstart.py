@@ -80,6 +80,13 @@ def test_kernel_start_follows_alpha_with_slopes(otf):
              float((err / scale).max()))
         assert numpy.all(it == 0)
         assert numpy.all(err <= 2e-5 * scale)
+    # alpha <= 0 keeps the coefficients; far alphas are clamped to 8x alpha_ref
+    u0a, _, _, _ = ctx.solve_tiles(d, g, 0.0, p)
+    assert numpy.all(numpy.isfinite(u0a))
+    assert numpy.array_equal(u0a, u_ref)
+    u_far, _, _, _ = ctx.solve_tiles(d, g, 50.0, p)
+    ref = linstart.predict(d, g, coef, alpha=0.8, slope=slope, alpha_ref=0.1)
+    assert numpy.allclose(u_far, ref, rtol=2e-5, atol=2e-5 * float(numpy.abs(ref).max()))
     # set_linear_start alone clears the slopes
     ctx.set_linear_start(coef)
     u_cleared, _, _, _ = ctx.solve_tiles(d, g, 0.3, p)

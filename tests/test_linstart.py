@@ -124,6 +124,10 @@ def test_predictor_at_the_reference_alpha_ignores_the_slopes():
     assert numpy.array_equal(linstart.predict(d, g, coef, alpha=linstart.ALPHA_REF,
                                               slope=slope), u0)
     assert not numpy.allclose(linstart.predict(d, g, coef, alpha=0.2, slope=slope), u0)
+    # alpha <= 0 keeps the coefficients; beyond 8x alpha_ref the slope is clamped
+    assert numpy.array_equal(linstart.predict(d, g, coef, alpha=0.0, slope=slope), u0)
+    assert numpy.array_equal(linstart.predict(d, g, coef, alpha=5.0, slope=slope),
+                             linstart.predict(d, g, coef, alpha=0.8, slope=slope))
 
 
 def test_stored_slopes_load():
