@@ -157,7 +157,10 @@ long long ncs_hip_tile_count_window(int height, int width, int geometry, int r);
  * host reduceNoise (pyOpenCLNCS/ncs.py:47-198): every frame may have its own
  * size, gamma, alpha and geometry and all tiles of all frames go through one
  * persistent launch.  iterations/status/cost (host, may be NULL) receive one
- * entry per tile, frames in order, tiles in raster order. */
+ * entry per tile, frames in order, tiles in raster order.  The launch
+ * schedules the frames smallest first (stable, by tile count), so that the
+ * long-running tiles of small frames and ROIs start early; results and the
+ * order of every per-tile array are the same as in the caller's order. */
 int ncs_hip_reduce_frames(ncs_ctx *ctx, const ncs_frame *frames, int n_frames,
                           const ncs_params *params, int *iterations,
                           int *status, float *cost);
@@ -179,7 +182,7 @@ int ncs_hip_reduce_frames_device(ncs_ctx *ctx, const ncs_frame *frames,
  * reference: for callers that cannot allocate device memory (code under HIP
  * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
  * work-queue counters and the frame table live in a fixed device array of
- * the library (19,456 bytes per GPU, held until the process ends), written
+ * the library (21,504 bytes per GPU, held until the process ends), written
  * by kernels; at most 256 frames per call; the learned (conv-net) start,
  * which needs a start buffer, is refused.  Every call on one GPU shares that
  * array: calls that overlap on one GPU race, whatever their context. */

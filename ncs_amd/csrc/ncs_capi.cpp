@@ -168,7 +168,7 @@ static hipError_t set_device(ncs_ctx *ctx) { return hipSetDevice(ctx->device); }
 
 // ncs_hip_reduce_frames_device_noalloc keeps its work-queue counters and its
 // frame table in this fixed array instead of context buffers: kQueueBytes
-// of counters, then up to kScratchFrames frame descriptors (19,456 bytes).
+// of counters, then up to kScratchFrames frame descriptors (21,504 bytes).
 // Each GPU holds its own copy until the process ends; every call on a GPU
 // shares it.
 constexpr int kScratchFrames = 256;
@@ -176,6 +176,7 @@ constexpr size_t kScratchFrameOffset = 1024;
 constexpr size_t kScratchBytes = kScratchFrameOffset + kScratchFrames * sizeof(FrameDev);
 static_assert(kQueueBytes <= kScratchFrameOffset, "queue counters overflow their slot");
 static_assert(kScratchFrameOffset % alignof(FrameDev) == 0, "misaligned frame table");
+static_assert(kScratchBytes == 21504, "the documented scratch size");
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
 constexpr int kMaxDevices = 64;
@@ -526,6 +527,7 @@ static int describe_frames(const ncs_ctx *ctx, const ncs_frame *frames, int n, F
     d.ty = (int)tiles_axis(f.width, f.geometry, win);
     d.win = win;
     d.first = total;
+    d.slot = total;
     total += nt;
   }
   if (total > 0xF0000000LL) return fail(NCS_E_ARG, "too many tiles in one batch");
@@ -536,6 +538,57 @@ static int build_frames(const ncs_ctx *ctx, const ncs_frame *frames, int n,
                         std::vector<FrameDev> &fd, long long &total) {
   fd.resize(n);
   return describe_frames(ctx, frames, n, fd.data(), total);
+}
+
+// Frame order of the schedule.  The solver claims tiles in frame-table order:
+// the first half in static per-wave blocks, the rest from per-XCD queues, so
+// the last tiles claimed set the launch's tail.  Frames are scheduled smallest
+// first (stable, by tile count): small frames and ROIs carry the tiles that
+// run long (edge blocks that are mostly padding, few valid pixels), and this
+// order gives them to many waves at the start, leaving ordinary full tiles of
+// the large frames for the end.  On a batch of equal frames it is the
+// caller's order.  Results do not depend on it: each tile is solved by one
+// wave on its own, and per-tile arrays stay in the caller's order (`slot`).
+// `fd` holds the frames in the caller's order (first == slot); `order` is the
+// schedule (indices into fd) and `out` receives the table in that order.
+static void place_frames(const FrameDev *fd, int n, long long total, const int *order,
+                         FrameDev *out) {
+  long long first = 0;
+  for (int k = 0; k < n; k++) {
+    const int i = order[k];
+    out[k] = fd[i];
+    out[k].first = first;
+    first += (i + 1 < n ? fd[i + 1].slot : total) - fd[i].slot;
+  }
+}
+
+static void schedule_small_first(std::vector<FrameDev> &fd, long long total) {
+  const int n = (int)fd.size();
+  std::vector<long long> nt(n);
+  for (int i = 0; i < n; i++) nt[i] = (i + 1 < n ? fd[i + 1].slot : total) - fd[i].slot;
+  std::vector<int> order(n);
+  for (int i = 0; i < n; i++) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return nt[a] < nt[b]; });
+  std::vector<FrameDev> out(n);
+  place_frames(fd.data(), n, total, order.data(), out.data());
+  fd.swap(out);
+}
+
+// The same order without allocating: a stable insertion sort over at most
+// kScratchFrames frames, in caller-provided arrays.
+static void schedule_small_first_fixed(const FrameDev *fd, int n, long long total, int *order,
+                                       FrameDev *out) {
+  auto tiles = [&](int i) { return (i + 1 < n ? fd[i + 1].slot : total) - fd[i].slot; };
+  for (int k = 0; k < n; k++) {
+    int j = k;
+    const long long key = tiles(k);
+    while (j > 0 && tiles(order[j - 1]) > key) {
+      order[j] = order[j - 1];
+      j--;
+    }
+    order[j] = k;
+  }
+  place_frames(fd, n, total, order, out);
 }
 
 // Frame descriptors go to the device through a pinned staging copy; a batch
@@ -682,6 +735,7 @@ extern "C" int ncs_hip_reduce_frames_device(ncs_ctx *ctx, const ncs_frame *frame
   HIPCHK(set_device(ctx));
   hipStream_t s = (hipStream_t)stream;  // 0 = the default (NULL) stream
   HIPCHK(order_after_previous(ctx, s));
+  schedule_small_first(fd, total);
   rc = upload_frames(ctx, fd, s);
   if (rc) return rc;
   const float *u0 = params->start == NCS_START_TILES ? u0_tiles : nullptr;
@@ -725,10 +779,12 @@ extern "C" int ncs_hip_reduce_frames_device_noalloc(ncs_ctx *ctx, const ncs_fram
   if (params->start == NCS_START_LINEAR && !ctx->has_linear)
     return fail(NCS_E_NOWEIGHTS, "linear start requested but no coefficients are set");
   if (!ctx->scratch) return fail(NCS_E_HIP, "no device scratch on this context's GPU");
-  FrameDev fd[kScratchFrames];
+  FrameDev caller[kScratchFrames], fd[kScratchFrames];
+  int order[kScratchFrames];
   long long total = 0;
-  rc = describe_frames(ctx, frames, n_frames, fd, total);
+  rc = describe_frames(ctx, frames, n_frames, caller, total);
   if (rc) return rc;
+  schedule_small_first_fixed(caller, n_frames, total, order, fd);
   HIPCHK(set_device(ctx));
   hipStream_t s = (hipStream_t)stream;
   unsigned int *dq = static_cast<unsigned int *>(ctx->scratch);
@@ -837,6 +893,7 @@ extern "C" int ncs_hip_reduce_frames(ncs_ctx *ctx, const ncs_frame *frames, int 
     fd[i].out = f.out ? dout + off[i] : nullptr;
     fd[i].ds = fd[i].gs = fd[i].os = f.width;
   }
+  schedule_small_first(fd, total);
   rc = upload_frames(ctx, fd, s);
   if (rc) return rc;
   rc = enqueue_solve(ctx, ctx->frames.as<FrameDev>(), n_frames, any_pad(ctx, frames, n_frames),
@@ -957,6 +1014,7 @@ extern "C" int ncs_hip_net_predict_frames_device(ncs_ctx *ctx, const ncs_frame *
   if (rc) return rc;
   HIPCHK(set_device(ctx));
   hipStream_t s = (hipStream_t)stream;
+  schedule_small_first(fd, total);
   rc = upload_frames(ctx, fd, s);
   if (rc) return rc;
   const int grid = net_forward_grid(total, ctx->cus);
@@ -1066,6 +1124,7 @@ extern "C" int ncs_hip_autotune(ncs_ctx *ctx, const ncs_frame *frames, int n_fra
   if (rc) return rc;
   if (total <= 0) return fail(NCS_E_ARG, "empty sample");
   for (FrameDev &f : fd) f.out = nullptr;  // time the solve, write nothing
+  schedule_small_first(fd, total);  // time the order the solves run in
   const bool pad = any_pad(ctx, frames, n_frames);
   HIPCHK(set_device(ctx));
   hipStream_t s = ctx->stream;
@@ -1292,6 +1351,7 @@ extern "C" int ncs_hip_pipeline_create(ncs_ctx *ctx, const ncs_pipeline_frame *f
     d.ty = (int)tiles_axis(f.width, f.geometry, ctx->win);
     d.win = ctx->win;
     d.first = pl->total;
+    d.slot = pl->total;
     pl->total += nt;
     if (f.geometry != NCS_GEOM_REFERENCE || ctx->win != 16) pl->pad = true;
   }
@@ -1318,6 +1378,7 @@ extern "C" int ncs_hip_pipeline_create(ncs_ctx *ctx, const ncs_pipeline_frame *f
     fd[i].gamma = pl->gamma.as<float>() + pl->off[i];
     fd[i].out = pl->out.as<float>() + pl->off[i];
   }
+  schedule_small_first(fd, pl->total);
   HIPCHK(pl->frames.ensure(fd.size() * sizeof(FrameDev)));
   HIPCHK(hipMemcpy(pl->frames.p, fd.data(), fd.size() * sizeof(FrameDev), hipMemcpyHostToDevice));
   HIPCHK(hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking));

@@ -40,7 +40,11 @@ constexpr int kTableSlope = 10;
 constexpr int kTableLaneFloats = 4 * kTableF4;
 constexpr int kTableFloats = kWave * kTableLaneFloats;
 
-// One frame of a batch, as the kernel sees it.
+// One frame of a batch, as the kernel sees it.  A batch's tiles are claimed
+// in frame-table order (`first`, increasing along the table); per-tile arrays
+// (warm starts, tile outputs, iterations, status, cost) are indexed in the
+// caller's frame order (`slot`).  The two differ when the host schedules the
+// frames in another order than the caller passed them.
 struct FrameDev {
   const float *data;
   const float *gamma;
@@ -52,7 +56,8 @@ struct FrameDev {
   int ty;              // tiles along the fast axis
   int win;             // window side r of the reference geometry (even, 4..16);
                        // a window fills the top-left r x r of the 16 x 16 tile
-  long long first;     // global index of the frame's first tile
+  long long first;     // schedule position of the frame's first tile
+  long long slot;      // per-tile array index of the frame's first tile
 };
 
 // Solver parameters in the kernel's units.

@@ -573,9 +573,11 @@ void net_warm_kernel(const FrameDev *__restrict__ frames, int n_frames, long lon
 #pragma unroll
     for (int r = 0; r < 4; r++) dkeep[r * 64 + lane] = d[r];
     const f4 y = net_body(io, pk, bl, lane, nullptr, 0, 0);
+    // starts are indexed in the caller's frame order
+    float *const w = u0 + (F.slot + local) * 256;
 #pragma unroll
     for (int r = 0; r < 4; r++)
-      u0[t * 256 + (4 * grp + r) * 16 + col] = dkeep[r * 64 + lane] + sigma * y[r];
+      w[(4 * grp + r) * 16 + col] = dkeep[r * 64 + lane] + sigma * y[r];
     wave_sync();
   }
 }

@@ -126,7 +126,7 @@ __device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x)
 // v_log_f32 (log2) times ln 2, two instructions.  The libm-accurate logf
 // adds denormal rescaling and an extended-precision ln 2 product (about a
 // dozen instructions) that the solver's arguments (>= 1e-6) never need.
-constexpr float kLn2 = 0.693147182f, kInvLn2 = 1.44269504f;
+constexpr float kLn2 = 0.693147182f;
 __device__ __forceinline__ float ln_hw(float x) {
   return __builtin_amdgcn_logf(x) * kLn2;
 }
@@ -920,7 +920,7 @@ void solve_kernel(SolveArgs A) {
       }
     }
     if (A.u0 != nullptr) {
-      const float *w = A.u0 + t * kTilePix;  // uniform
+      const float *w = A.u0 + (F.slot + (t - F.first)) * kTilePix;  // uniform
 #pragma unroll
       for (int r = 0; r < 4; r++) w0[r] = w[(unsigned int)((4 * grp + r) * kTile + col)];
     }
@@ -1410,11 +1410,13 @@ void solve_kernel(SolveArgs A) {
       f = red[0] + 0.5f * alpha * red[1];
     }
 
-    // stitch the owned interior and per-tile results
+    // stitch the owned interior and per-tile results (per-tile arrays in the
+    // caller's frame order)
     int ln = lane;
     asm volatile("" : "+v"(ln));
     const int col = ln & 15, grp = ln >> 4;
-    float *const ut_base = A.u_tiles != nullptr ? A.u_tiles + t * kTilePix : nullptr;
+    const long long ts = F.slot + (t - F.first);
+    float *const ut_base = A.u_tiles != nullptr ? A.u_tiles + ts * kTilePix : nullptr;
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       const int row = 4 * grp + r;
@@ -1424,10 +1426,10 @@ void solve_kernel(SolveArgs A) {
       if (ut_base != nullptr) ut_base[(unsigned int)(row * kTile + col)] = u[r];
     }
     if (lane == 0) {
-      if (A.iterations) A.iterations[t] = iters;
-      if (A.status) A.status[t] = status;
-      if (A.cost) A.cost[t] = f;
-      if (A.evaluations) A.evaluations[t] = evals;
+      if (A.iterations) A.iterations[ts] = iters;
+      if (A.status) A.status[ts] = status;
+      if (A.cost) A.cost[ts] = f;
+      if (A.evaluations) A.evaluations[ts] = evals;
     }
     if (tn < 0) break;
     t = tn;
