@@ -550,45 +550,41 @@ static int build_frames(const ncs_ctx *ctx, const ncs_frame *frames, int n,
 // caller's order.  Results do not depend on it: each tile is solved by one
 // wave on its own, and per-tile arrays stay in the caller's order (`slot`).
 // `fd` holds the frames in the caller's order (first == slot); `order` is the
-// schedule (indices into fd) and `out` receives the table in that order.
-static void place_frames(const FrameDev *fd, int n, long long total, const int *order,
-                         FrameDev *out) {
-  long long first = 0;
-  for (int k = 0; k < n; k++) {
-    const int i = order[k];
-    out[k] = fd[i];
-    out[k].first = first;
-    first += (i + 1 < n ? fd[i + 1].slot : total) - fd[i].slot;
-  }
+// schedule (indices into fd).
+static long long frame_tiles(const FrameDev *fd, int n, long long total, int i) {
+  return (i + 1 < n ? fd[i + 1].slot : total) - fd[i].slot;
 }
 
 static void schedule_small_first(std::vector<FrameDev> &fd, long long total) {
   const int n = (int)fd.size();
   std::vector<long long> nt(n);
-  for (int i = 0; i < n; i++) nt[i] = (i + 1 < n ? fd[i + 1].slot : total) - fd[i].slot;
+  for (int i = 0; i < n; i++) nt[i] = frame_tiles(fd.data(), n, total, i);
   std::vector<int> order(n);
   for (int i = 0; i < n; i++) order[i] = i;
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return nt[a] < nt[b]; });
   std::vector<FrameDev> out(n);
-  place_frames(fd.data(), n, total, order.data(), out.data());
+  long long first = 0;
+  for (int k = 0; k < n; k++) {
+    out[k] = fd[order[k]];
+    out[k].first = first;
+    first += nt[order[k]];
+  }
   fd.swap(out);
 }
 
 // The same order without allocating: a stable insertion sort over at most
-// kScratchFrames frames, in caller-provided arrays.
-static void schedule_small_first_fixed(const FrameDev *fd, int n, long long total, int *order,
-                                       FrameDev *out) {
-  auto tiles = [&](int i) { return (i + 1 < n ? fd[i + 1].slot : total) - fd[i].slot; };
+// kScratchFrames frames into the caller-provided `order`; the caller places
+// frame order[k] at schedule position k.
+static void schedule_small_first_fixed(const FrameDev *fd, int n, long long total, int *order) {
   for (int k = 0; k < n; k++) {
     int j = k;
-    const long long key = tiles(k);
-    while (j > 0 && tiles(order[j - 1]) > key) {
+    const long long key = frame_tiles(fd, n, total, k);
+    while (j > 0 && frame_tiles(fd, n, total, order[j - 1]) > key) {
       order[j] = order[j - 1];
       j--;
     }
     order[j] = k;
   }
-  place_frames(fd, n, total, order, out);
 }
 
 // Frame descriptors go to the device through a pinned staging copy; a batch
@@ -779,22 +775,28 @@ extern "C" int ncs_hip_reduce_frames_device_noalloc(ncs_ctx *ctx, const ncs_fram
   if (params->start == NCS_START_LINEAR && !ctx->has_linear)
     return fail(NCS_E_NOWEIGHTS, "linear start requested but no coefficients are set");
   if (!ctx->scratch) return fail(NCS_E_HIP, "no device scratch on this context's GPU");
-  FrameDev caller[kScratchFrames], fd[kScratchFrames];
+  FrameDev fd[kScratchFrames];
   int order[kScratchFrames];
   long long total = 0;
-  rc = describe_frames(ctx, frames, n_frames, caller, total);
+  rc = describe_frames(ctx, frames, n_frames, fd, total);
   if (rc) return rc;
-  schedule_small_first_fixed(caller, n_frames, total, order, fd);
+  schedule_small_first_fixed(fd, n_frames, total, order);
   HIPCHK(set_device(ctx));
   hipStream_t s = (hipStream_t)stream;
   unsigned int *dq = static_cast<unsigned int *>(ctx->scratch);
   FrameDev *dfr =
       reinterpret_cast<FrameDev *>(static_cast<unsigned char *>(ctx->scratch) + kScratchFrameOffset);
+  long long first = 0;  // schedule position of the next frame's first tile
   for (int i0 = 0; i0 < n_frames; i0 += kFramesPerWrite) {
     FrameChunk c;
     c.first = i0;
     c.n = std::min(kFramesPerWrite, n_frames - i0);
-    for (int k = 0; k < c.n; k++) c.f[k] = fd[i0 + k];
+    for (int k = 0; k < c.n; k++) {
+      const int i = order[i0 + k];
+      c.f[k] = fd[i];
+      c.f[k].first = first;
+      first += frame_tiles(fd, n_frames, total, i);
+    }
     hipLaunchKernelGGL(write_frames_kernel, dim3(1), dim3(64), 0, s, c, dfr);
     HIPCHK(hipGetLastError());
   }
